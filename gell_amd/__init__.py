@@ -1,0 +1,31 @@
+"""MI355X (gfx950) implementation of the Gell per-step hot path.
+
+Mechanics (Morton cell list, radix sort, adhesion/repulsion forces, AB2
+movement), LOD diffusion (x/y/z Thomas sweeps) and proliferation/death
+(Philox-driven division, stream compaction) run as hand-written HIP kernels
+behind the C ABI of ``include/gell_abi.h``; ``gell_amd.runtime`` binds that
+ABI from Python.
+"""
+from .runtime import (  # noqa: F401
+    CELL_DTYPE,
+    Context,
+    DeviceBuffer,
+    GellError,
+    Params,
+    default_params,
+    float_loop_count,
+    load_library,
+    make_params,
+)
+
+__all__ = [
+    "CELL_DTYPE",
+    "Context",
+    "DeviceBuffer",
+    "GellError",
+    "Params",
+    "default_params",
+    "float_loop_count",
+    "load_library",
+    "make_params",
+]

@@ -1,0 +1,442 @@
+// fm.hip -- the mechanics step on gfx950 (MechanicsMesh_struct::FM_update,
+// FMkernel.cuh:181-202).
+//
+// Reference: key kernel -> thrust::sort_by_key of the 72-byte records ->
+// atomic min/max cell ranges (after a full reset of all N^3 ranges) -> force
+// kernel -> movement kernel. Here:
+//   fm_keys      Morton key per slot (biased by +1 so that the outside
+//                sentinel -1 becomes 0 and still sorts first); clears the
+//                ranges the previous call wrote (O(n), not O(N^3)).
+//   radix_*      stable LSD radix sort of (key, slot) pairs: per pass a tile
+//                histogram, a per-digit column scan, and a scatter whose
+//                stable in-tile rank comes from wave64 ballots.
+//   fm_gather    one 72-byte gather of the records into key order; applies the
+//                key kernel's record updates (sign, mesh_idx), builds the
+//                per-voxel live ranges with atomic min/max (order-independent
+//                result) and the compact (x, y, z, r) neighbour array.
+//   fm_force_move  forcekernel + movementkernel fused: reads the sorted
+//                record once, visits the 27-voxel stencil in the reference's
+//                order (x outer, z inner, ascending slot) through the compact
+//                array, moves the cell and writes the record back in place.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace gell {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Keys
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fm_keys(const gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
+                                                  uint32_t* __restrict__ keys, const int32_t* __restrict__ clear_list,
+                                                  int32_t clear_n, int2* __restrict__ se, int32_t s_empty) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < clear_n) {
+        const int32_t v = clear_list[i];
+        if (v >= 0) se[v] = make_int2(s_empty, -1);
+    }
+    if (i < n) {
+        const float* pos = reinterpret_cast<const float*>(cells + i) + W_PX;
+        const int32_t a = trunc_index(pos[0] / dx), b = trunc_index(pos[1] / dx), c = trunc_index(pos[2] / dx);
+        keys[i] = inside(a, b, c, N) ? morton3(a, b, c) + 1u : 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of one int per thread (256 threads = 4 waves).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_inclusive_scan(int x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int block_exclusive_scan(int x, int* lds4, int& total) {
+    const int lane = lane_id(), w = threadIdx.x / 64;
+    const int inc = wave_inclusive_scan(x);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+        const int s = lds4[q];
+        if (q < w) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - x;
+}
+
+// Lanes of the wave whose digit equals this lane's (BITS ballots).
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+    uint64_t peers = valid;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort passes
+// ---------------------------------------------------------------------------
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void radix_hist(const uint32_t* __restrict__ keys, int32_t n, int shift,
+                                                           int32_t* __restrict__ hist, int nblocks) {
+    constexpr int BINS = 1 << BITS;
+    __shared__ int h[BINS];
+    for (int b = threadIdx.x; b < BINS; b += kSortThreads) h[b] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const int lane = lane_id();
+#pragma unroll 4
+    for (int it = 0; it < kSortItems; ++it) {
+        const int64_t idx = base + (int64_t)it * kSortThreads + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & (BINS - 1) : 0u;
+        const uint64_t vmask = __ballot(valid);
+        const uint64_t peers = match_digit<BITS>(d, vmask);
+        if (valid && lane == 63 - __clzll(peers)) atomicAdd(&h[d], __popcll(peers));
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < BINS; b += kSortThreads) hist[(int64_t)b * nblocks + blockIdx.x] = h[b];
+}
+
+// One workgroup per digit: exclusive scan of that digit's per-tile counts.
+__global__ __launch_bounds__(kSortThreads) void radix_colscan(int32_t* __restrict__ hist, int nblocks,
+                                                              int32_t* __restrict__ tot) {
+    __shared__ int lds4[kSortThreads / 64];
+    int32_t* col = hist + (int64_t)blockIdx.x * nblocks;
+    int carry = 0;
+    for (int base = 0; base < nblocks; base += kSortThreads * 4) {
+        int v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + threadIdx.x * 4 + q;
+            v[q] = idx < nblocks ? col[idx] : 0;
+            s += v[q];
+        }
+        int total;
+        int ex = block_exclusive_scan(s, lds4, total) + carry;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + threadIdx.x * 4 + q;
+            if (idx < nblocks) col[idx] = ex;
+            ex += v[q];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint32_t* __restrict__ kin,
+                                                              const int32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                                              int32_t* __restrict__ signed_out, int32_t n, int shift,
+                                                              const int32_t* __restrict__ hist,
+                                                              const int32_t* __restrict__ tot, int nblocks) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int PER = (BINS + kSortThreads - 1) / kSortThreads;
+    constexpr int NW = kSortThreads / 64;
+    __shared__ int cnt[NW][BINS];
+    __shared__ int lbase[BINS];
+    __shared__ int gbase[BINS];
+    __shared__ uint32_t sk[kSortTile];
+    __shared__ int32_t sv[kSortTile];
+    __shared__ int lds4[NW];
+
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / 64;
+    const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+    const int tile_n = (int)min<int64_t>(kSortTile, (int64_t)n - tile0);
+
+    // Global base of each digit for this tile: digits below + earlier tiles.
+    {
+        int v[PER], s = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int b = tid * PER + q;
+            v[q] = b < BINS ? tot[b] : 0;
+            s += v[q];
+        }
+        int total;
+        int ex = block_exclusive_scan(s, lds4, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int b = tid * PER + q;
+            if (b < BINS) gbase[b] = ex + hist[(int64_t)b * nblocks + blockIdx.x];
+            ex += v[q];
+        }
+    }
+    for (int b = tid; b < BINS * NW; b += kSortThreads) (&cnt[0][0])[b] = 0;
+    __syncthreads();
+
+    // Stable rank inside each wave's contiguous sub-tile.
+    uint32_t k[kSortItems];
+    int32_t val[kSortItems];
+    int rk[kSortItems];
+    const int64_t wbase = tile0 + (int64_t)w * 64 * kSortItems;
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int64_t idx = wbase + it * 64 + lane;
+        const bool valid = idx < n;
+        k[it] = valid ? kin[idx] : 0u;
+        val[it] = valid ? (vin ? vin[idx] : (int32_t)idx) : 0;
+        const uint32_t d = (k[it] >> shift) & (BINS - 1);
+        const uint64_t vmask = __ballot(valid);
+        const uint64_t peers = match_digit<BITS>(d, vmask);
+        const int before = valid ? cnt[w][d] : 0;
+        rk[it] = before + __popcll(peers & lanemask_lt());
+        if (valid && lane == 63 - __clzll(peers)) cnt[w][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+
+    // Per digit: offsets of the waves, and the tile-local digit base.
+    {
+        int v[PER], s = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int b = tid * PER + q;
+            int acc = 0;
+            if (b < BINS) {
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) {
+                    const int c = cnt[ww][b];
+                    cnt[ww][b] = acc;
+                    acc += c;
+                }
+            }
+            v[q] = acc;
+            s += acc;
+        }
+        int total;
+        int ex = block_exclusive_scan(s, lds4, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int b = tid * PER + q;
+            if (b < BINS) lbase[b] = ex;
+            ex += v[q];
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int64_t idx = wbase + it * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (k[it] >> shift) & (BINS - 1);
+            const int pos = lbase[d] + cnt[w][d] + rk[it];
+            sk[pos] = k[it];
+            sv[pos] = val[it];
+        }
+    }
+    __syncthreads();
+
+    for (int q = tid; q < tile_n; q += kSortThreads) {
+        const uint32_t kk = sk[q];
+        const uint32_t d = (kk >> shift) & (BINS - 1);
+        const int64_t g = gbase[d] + (q - lbase[d]);
+        kout[g] = kk;
+        vout[g] = sv[q];
+        if (signed_out) signed_out[g] = (int32_t)kk - 1;
+    }
+}
+
+template <int BITS>
+hipError_t sort_pass(gell_ctx* ctx, const uint32_t* kin, const int32_t* vin, uint32_t* kout, int32_t* vout,
+                     int32_t* signed_out, int32_t n, int shift) {
+    const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL((radix_hist<BITS>), dim3(nblocks), dim3(kSortThreads), 0, st, kin, n, shift, ctx->hist,
+                       nblocks);
+    hipLaunchKernelGGL(radix_colscan, dim3(1 << BITS), dim3(kSortThreads), 0, st, ctx->hist, nblocks, ctx->hist_tot);
+    hipLaunchKernelGGL((radix_scatter<BITS>), dim3(nblocks), dim3(kSortThreads), 0, st, kin, vin, kout, vout,
+                       signed_out, n, shift, ctx->hist, ctx->hist_tot, nblocks);
+    return hipGetLastError();
+}
+
+typedef hipError_t (*PassFn)(gell_ctx*, const uint32_t*, const int32_t*, uint32_t*, int32_t*, int32_t*, int32_t, int);
+const PassFn kPass[10] = {nullptr,         sort_pass<1>, sort_pass<2>, sort_pass<3>, sort_pass<4>,
+                          sort_pass<5>,    sort_pass<6>, sort_pass<7>, sort_pass<8>, sort_pass<9>};
+
+// ---------------------------------------------------------------------------
+// Gather + cell list
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict__ cells,
+                                                    const int32_t* __restrict__ order, int32_t n, int32_t N, float dx,
+                                                    gell_cell* __restrict__ sorted, float4* __restrict__ nb,
+                                                    int2* __restrict__ se, int32_t* __restrict__ se_list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    CellRegs c = load_cell(cells, order[i]);
+    const float px = cf(c, W_PX), py = cf(c, W_PY), pz = cf(c, W_PZ);
+    const int32_t a = trunc_index(px / dx), b = trunc_index(py / dx), k = trunc_index(pz / dx);
+    // updateCellMeshIdx's record writes (FMkernel.cuh:26-35).
+    if (!inside(a, b, k, N))
+        cset_sign(c, false);
+    else
+        c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
+    store_cell(sorted, i, c);
+    const bool live = csign(c);
+    const int32_t v = live ? (int32_t)c.w[W_MESH] : -1;
+    se_list[i] = v;
+    nb[i] = make_float4(px, py, pz, live ? cf(c, W_R) : __uint_as_float(kDeadTag));
+    if (live) {
+        // Mech_Mesh_se_Get, FMkernel.cuh:45-53: min/max of the live slots.
+        atomicMin(&se[v].x, (int)i);
+        atomicMax(&se[v].y, (int)i);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Force + movement
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
+                                                        const float4* __restrict__ nb, const int2* __restrict__ se,
+                                                        gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
+                                                        float dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    CellRegs c = load_cell(sorted, i);
+    if (csign(c)) {
+        const float p1x = cf(c, W_PX), p1y = cf(c, W_PY), p1z = cf(c, W_PZ);
+        const float r1 = cf(c, W_R);
+        const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+        for (int xi = px - 1; xi <= px + 1; ++xi)
+            for (int yi = py - 1; yi <= py + 1; ++yi)
+                for (int zi = pz - 1; zi <= pz + 1; ++zi) {
+                    if (!inside(xi, yi, zi, N)) continue;
+                    const int2 r = se[lin(xi, yi, zi, N)];
+                    if (r.y < 0) continue;
+                    for (int j = r.x; j <= r.y; ++j) {
+                        const float4 q = nb[j];
+                        if (__float_as_uint(q.w) == kDeadTag) continue;
+                        // FMkernel.cuh:89-109
+                        const float ddx = p1x - q.x, ddy = p1y - q.y, ddz = p1z - q.z;
+                        const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const float distance = sqrtf((float)((double)ss + 0.0001));
+                        if (distance >= thres) continue;
+                        const float rsum = q.w + r1;
+                        const float R = rsum > 0.01f ? rsum : 0.01f;
+                        thres = (float)((double)R * 1.25);
+                        if (distance >= thres) continue;
+                        const float qr = 1.f - distance / R;
+                        const float qa = 1.f - distance / thres;
+                        const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
+                        const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
+                        const float cc = coefr - coefa;
+                        fx += ddx * cc;
+                        fy += ddy * cc;
+                        fz += ddz * cc;
+                    }
+                }
+        // movementkernel, FMkernel.cuh:130-158
+        const float h = dt / 2.f;
+        const float ox = cf(c, W_OX), oy = cf(c, W_OY), oz = cf(c, W_OZ);
+        const float nx = p1x + h * (3.f * fx - ox);
+        const float ny = p1y + h * (3.f * fy - oy);
+        const float nz = p1z + h * (3.f * fz - oz);
+        const int32_t a = trunc_index(nx / dx), b = trunc_index(ny / dx), k = trunc_index(nz / dx);
+        if (!inside(a, b, k, N)) {
+            cset_sign(c, false);
+            cset(c, W_FX, fx);
+            cset(c, W_FY, fy);
+            cset(c, W_FZ, fz);
+        } else {
+            cset(c, W_PX, nx);
+            cset(c, W_PY, ny);
+            cset(c, W_PZ, nz);
+            cset(c, W_OX, fx);
+            cset(c, W_OY, fy);
+            cset(c, W_OZ, fz);
+            cset(c, W_FX, 0.f);
+            cset(c, W_FY, 0.f);
+            cset(c, W_FZ, 0.f);
+            c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
+        }
+    }
+    store_cell(cells, i, c);
+}
+
+__global__ void se_fill(int2* se, int64_t V, int32_t s_empty) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) se[v] = make_int2(s_empty, -1);
+}
+
+}  // namespace
+
+SortPlan plan_sort(int key_bits) {
+    SortPlan p{};
+    p.passes = key_bits <= 9 ? 1 : (key_bits + 8) / 9;
+    int shift = 0;
+    for (int q = 0; q < p.passes; ++q) {
+        p.bits[q] = key_bits / p.passes + (q < key_bits % p.passes ? 1 : 0);
+        p.shift[q] = shift;
+        shift += p.bits[q];
+    }
+    return p;
+}
+
+hipError_t fm_init_ranges(gell_ctx* ctx) {
+    const unsigned blocks = (unsigned)((ctx->V + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(se_fill, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, ctx->V, ctx->P.capacity + 1);
+    ctx->se_list_n = 0;
+    return hipGetLastError();
+}
+
+hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
+    const int32_t m = n > ctx->se_list_n ? n : ctx->se_list_n;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(fm_keys, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells, n, ctx->N,
+                       ctx->P.dx, ctx->keys[0], ctx->se_list, ctx->se_list_n, ctx->se, ctx->P.capacity + 1);
+    ctx->se_list_n = 0;
+    return hipGetLastError();
+}
+
+hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
+    if (n == 0) return hipSuccess;
+    const SortPlan plan = plan_sort(3 * ceil_log2(ctx->N) + 1);
+    int cur = 0;
+    for (int q = 0; q < plan.passes; ++q) {
+        const bool last = q == plan.passes - 1;
+        const int32_t* vin = q == 0 ? nullptr : ctx->vals[cur];
+        int32_t* vout = last ? ctx->out_order : ctx->vals[1 - cur];
+        hipError_t e = kPass[plan.bits[q]](ctx, ctx->keys[cur], vin, ctx->keys[1 - cur], vout,
+                                           last ? ctx->out_keys : nullptr, n, plan.shift[q]);
+        if (e != hipSuccess) return e;
+        cur = 1 - cur;
+    }
+    return hipSuccess;
+}
+
+hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fm_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
+                       ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ctx->se, ctx->se_list);
+    ctx->se_list_n = n;
+    return hipGetLastError();
+}
+
+hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fm_force_move, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->sorted,
+                       ctx->nb, ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    return hipGetLastError();
+}
+
+}  // namespace gell

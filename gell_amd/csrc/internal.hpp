@@ -1,0 +1,97 @@
+// internal.hpp -- the context object behind the C ABI and the launchers each
+// kernel file provides. Host-side, not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gell_abi.h"
+
+struct gell_ctx {
+    gell_params P{};
+    hipStream_t stream = nullptr;
+    int dev = 0;
+    int32_t N = 0;
+    int64_t V = 0;
+    std::string err;
+
+    // Mechanics: sort buffers, cell list, compact neighbour array.
+    uint32_t* keys[2] = {nullptr, nullptr};
+    int32_t* vals[2] = {nullptr, nullptr};
+    int32_t* hist = nullptr;       // radix histograms, digit-major
+    int32_t* hist_tot = nullptr;   // per-digit totals
+    int64_t hist_cap = 0;          // entries of hist
+    int32_t* out_keys = nullptr;   // sorted signed keys of the last fm_update
+    int32_t* out_order = nullptr;  // permutation of the last fm_update
+    gell_cell* sorted = nullptr;   // records in key order
+    float4* nb = nullptr;          // (x, y, z, r or dead tag) per sorted slot
+    int2* se = nullptr;            // per voxel (s, e)
+    int32_t* se_list = nullptr;    // voxels written by the last fm_update
+    int32_t se_list_n = 0;
+
+    // Diffusion, per substrate.
+    float* field[GELL_MAX_SUBSTRATES] = {};
+    float* rate[GELL_MAX_SUBSTRATES] = {};
+    float* E[GELL_MAX_SUBSTRATES] = {};
+    float* F[GELL_MAX_SUBSTRATES] = {};
+    float side[GELL_MAX_SUBSTRATES] = {};
+    long long* acc[GELL_MAX_SUBSTRATES] = {};
+    int32_t* rate_list[GELL_MAX_SUBSTRATES] = {};
+    int32_t rate_list_n[GELL_MAX_SUBSTRATES] = {};
+
+    // Proliferation / death.
+    int32_t* flags = nullptr;      // per slot
+    int32_t* blk = nullptr;        // per-block counts, then offsets
+    int32_t* slots = nullptr;      // daughter slot per mother
+    int32_t* d_total = nullptr;    // device scalar
+    int32_t* h_total = nullptr;    // pinned host scalar
+    gell_cell* compact = nullptr;  // death scratch
+    int64_t blk_cap = 0;
+
+    // Profiling.
+    bool prof = false;
+    struct Mark {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    double prof_ms[GELL_N_PHASES] = {};
+    int64_t prof_calls[GELL_N_PHASES] = {};
+};
+
+namespace gell {
+
+enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, PH_LOD, PH_BIRTH, PH_DEATH };
+
+// Each launcher enqueues on ctx->stream and returns a hipError_t.
+hipError_t lod_substep_launch(gell_ctx* ctx, int s);
+hipError_t lod_init_coeffs(gell_ctx* ctx, int s);
+
+hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
+hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n);
+hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
+hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n);
+hipError_t fm_init_ranges(gell_ctx* ctx);
+
+hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n);
+
+hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out);
+hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out);
+
+// Radix-sort planning: number of passes and bits per pass for a key width.
+struct SortPlan {
+    int passes;
+    int bits[8];
+    int shift[8];
+};
+SortPlan plan_sort(int key_bits);
+
+inline int ceil_log2(int v) {
+    int b = 0;
+    while ((1 << b) < v) ++b;
+    return b;
+}
+
+}  // namespace gell

@@ -1,0 +1,76 @@
+// prelod.hip -- uptake map of one substrate (Mesh_struct::PreLOD,
+// LODkernel.cuh:270-278; oxy_consumption_update, LODkernel.cuh:12-22).
+//
+// The reference zeroes all N^3 rates and float-atomicAdds each live cell's
+// Cell::O2_consume() into its voxel, so its sums depend on the order the
+// atomics land in. Here every rate is turned into an exact 2^-32 fixed-point
+// integer and summed with 64-bit integer atomics (order-independent), then
+// rounded to float once. Only voxels that hold cells are touched: the voxels
+// the previous call wrote are cleared first (O(n) instead of O(N^3)).
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace gell {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void prelod_clear(const int32_t* __restrict__ list, int32_t m,
+                                                       long long* __restrict__ acc, float* __restrict__ rate) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t v = list[i];
+    if (v >= 0) {
+        acc[v] = 0;
+        rate[v] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void prelod_accumulate(const gell_cell* __restrict__ cells, int32_t n,
+                                                            int64_t V, float dx, float uptake, float dead_factor,
+                                                            long long* __restrict__ acc, int32_t* __restrict__ list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(cells + i);
+    const bool live = (w[W_SIGN] & 0xFFu) != 0;
+    const int32_t v = (int32_t)w[W_MESH];
+    int32_t out = -1;
+    if (live && v >= 0 && v < V) {
+        const float rate = cell_uptake(__uint_as_float(w[W_V]), (int32_t)w[W_PHASE], dx, uptake, dead_factor);
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + v), (unsigned long long)to_fixed(rate));
+        out = v;
+    }
+    list[i] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void prelod_finalize(const int32_t* __restrict__ list, int32_t n,
+                                                          const long long* __restrict__ acc,
+                                                          float* __restrict__ rate) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    if (v >= 0) rate[v] = from_fixed(acc[v]);
+}
+
+}  // namespace
+
+hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n) {
+    hipStream_t st = ctx->stream;
+    const int32_t m = ctx->rate_list_n[s];
+    if (m > 0)
+        hipLaunchKernelGGL(prelod_clear, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ctx->rate_list[s], m,
+                           ctx->acc[s], ctx->rate[s]);
+    ctx->rate_list_n[s] = 0;
+    if (n > 0) {
+        const gell_substrate& sb = ctx->P.sub[s];
+        const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(prelod_accumulate, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->V, ctx->P.dx,
+                           sb.uptake, sb.dead_factor, ctx->acc[s], ctx->rate_list[s]);
+        hipLaunchKernelGGL(prelod_finalize, dim3(blocks), dim3(kBlock), 0, st, ctx->rate_list[s], n, ctx->acc[s],
+                           ctx->rate[s]);
+        ctx->rate_list_n[s] = n;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gell

@@ -1,0 +1,347 @@
+"""ctypes binding of the C ABI in ``include/gell_abi.h``.
+
+Python-side mirror of the reference's host interface for the per-step path:
+``Context`` owns what ``MechanicsMesh_struct`` and ``Mesh_struct`` own in the
+reference (FMkernel.cuh:160-212, LODkernel.cuh:192-319) and exposes the five
+operations Main.cu's time loop calls (Main.cu:108-127). Device memory is
+handled through the library's own allocation helpers, so no other GPU
+binding is needed.
+
+The HIP library is required: if ``gell_amd/_build/libgell_hip.so`` is missing
+every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "libgell_hip.so"
+MAX_SUBSTRATES = 4
+N_PHASES = 8
+PHASE_NAMES = ["fm_keys", "fm_sort", "fm_gather", "fm_force_move", "prelod", "lod", "birth", "death"]
+
+# The 72-byte record of include/gell_abi.h (reference Cells.cuh:9-47).
+CELL_DTYPE = np.dtype(
+    [
+        ("sign", "u1"),
+        ("pad_", "u1", (3,)),
+        ("mesh_idx", "<i4"),
+        ("pos", "<f4", (3,)),
+        ("force", "<f4", (3,)),
+        ("oldforce", "<f4", (3,)),
+        ("r", "<f4"),
+        ("V", "<f4"),
+        ("Vf", "<f4"),
+        ("Vcs", "<f4"),
+        ("Vns", "<f4"),
+        ("phase", "<i4"),
+        ("cell_clock", "<f4"),
+    ]
+)
+assert CELL_DTYPE.itemsize == 72
+
+
+class Substrate(C.Structure):
+    _fields_ = [
+        ("boundary", C.c_float),
+        ("D", C.c_float),
+        ("decay", C.c_float),
+        ("uptake", C.c_float),
+        ("dead_factor", C.c_float),
+        ("initial", C.c_float),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32),
+        ("dx", C.c_float),
+        ("capacity", C.c_int32),
+        ("n_substrates", C.c_int32),
+        ("dt_bio", C.c_float),
+        ("dt_mech", C.c_float),
+        ("dt_diff", C.c_float),
+        ("reserved_", C.c_int32),
+        ("seed", C.c_uint64),
+        ("sub", Substrate * MAX_SUBSTRATES),
+    ]
+
+
+class GellError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Load the gfx950 library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise GellError(f"HIP library not built: {LIB_PATH} (run `python -m gell_amd.build`)")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i32, i64, u64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_size_t
+    P = C.POINTER
+    sig = {
+        "gell_default_params": (None, [P(Params)]),
+        "gell_create": (C.c_int, [P(Params), vp, P(vp)]),
+        "gell_destroy": (None, [vp]),
+        "gell_last_error": (C.c_char_p, [vp]),
+        "gell_status_string": (C.c_char_p, [C.c_int]),
+        "gell_fm_update": (C.c_int, [vp, vp, i32]),
+        "gell_prelod": (C.c_int, [vp, i32, vp, i32]),
+        "gell_lod_substep": (C.c_int, [vp, i32]),
+        "gell_birth": (C.c_int, [vp, vp, i32, u64, P(i32)]),
+        "gell_death": (C.c_int, [vp, vp, i32, P(i32)]),
+        "gell_field": (vp, [vp, i32]),
+        "gell_uptake_field": (vp, [vp, i32]),
+        "gell_sorted_keys": (vp, [vp]),
+        "gell_sorted_order": (vp, [vp]),
+        "gell_voxel_ranges": (vp, [vp]),
+        "gell_birth_slots": (vp, [vp]),
+        "gell_sync": (C.c_int, [vp]),
+        "gell_profile_enable": (C.c_int, [vp, C.c_int]),
+        "gell_profile_read": (C.c_int, [vp, P(C.c_double), P(i64), C.c_int]),
+        "gell_dev_alloc": (vp, [sz]),
+        "gell_dev_free": (None, [vp]),
+        "gell_copy_h2d": (C.c_int, [vp, vp, sz]),
+        "gell_copy_d2h": (C.c_int, [vp, vp, sz]),
+        "gell_copy_d2d": (C.c_int, [vp, vp, sz]),
+        "gell_device_sync": (C.c_int, []),
+        "gell_device_name": (C.c_int, [C.c_char_p, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names every build of the library must export (those of gell_abi.h)."""
+    hdr = Path(__file__).resolve().parent.parent / "include" / "gell_abi.h"
+    import re
+
+    names = []
+    for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(gell_\w+)\s*\(", hdr.read_text(), re.M):
+        names.append(m.group(1))
+    return sorted(set(names))
+
+
+def default_params(**overrides) -> Params:
+    """Params filled with the reference's Definitions.h values, then overrides.
+
+    ``substrates`` may be a list of dicts with Substrate field names.
+    """
+    p = Params()
+    load_library().gell_default_params(C.byref(p))
+    subs = overrides.pop("substrates", None)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    if subs is not None:
+        p.n_substrates = len(subs)
+        for s, d in enumerate(subs):
+            for k, v in d.items():
+                setattr(p.sub[s], k, v)
+    return p
+
+
+def make_params(**kw) -> Params:
+    """Params without touching the library (for CPU-only code paths)."""
+    p = Params()
+    p.N, p.dx, p.capacity, p.n_substrates = 100, 25.0, 2_000_000, 1
+    p.dt_bio, p.dt_mech, p.dt_diff, p.seed = 6.0, 0.1, 0.01, 123
+    p.sub[0].boundary, p.sub[0].D, p.sub[0].decay = 38.0, 1e5, 0.1
+    p.sub[0].uptake, p.sub[0].dead_factor, p.sub[0].initial = -10.0, 0.1, 38.0
+    subs = kw.pop("substrates", None)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    if subs is not None:
+        p.n_substrates = len(subs)
+        for s, d in enumerate(subs):
+            for k, v in d.items():
+                setattr(p.sub[s], k, v)
+    return p
+
+
+class DeviceBuffer:
+    """A device allocation made through the library."""
+
+    def __init__(self, nbytes: int):
+        self.lib = load_library()
+        self.nbytes = int(nbytes)
+        self.ptr = self.lib.gell_dev_alloc(max(1, self.nbytes))
+        if not self.ptr:
+            raise GellError(f"device allocation of {nbytes} bytes failed")
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        if a.nbytes and self.lib.gell_copy_h2d(self.ptr + offset, a.ctypes.data, a.nbytes) != 0:
+            raise GellError("host-to-device copy failed")
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        if out.nbytes and self.lib.gell_copy_d2h(out.ctypes.data, self.ptr + offset, out.nbytes) != 0:
+            raise GellError("device-to-host copy failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.lib.gell_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_ptr(ptr: int, dtype, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=dtype)
+    if out.nbytes and load_library().gell_copy_d2h(out.ctypes.data, ptr, out.nbytes) != 0:
+        raise GellError("device-to-host copy failed")
+    return out
+
+
+def upload_ptr(ptr: int, a: np.ndarray) -> None:
+    a = np.ascontiguousarray(a)
+    if a.nbytes and load_library().gell_copy_h2d(ptr, a.ctypes.data, a.nbytes) != 0:
+        raise GellError("host-to-device copy failed")
+
+
+class Context:
+    """One simulation context: cell-list, fields, uptake maps, scratch."""
+
+    def __init__(self, params: Params, stream: int | None = None):
+        self.lib = load_library()
+        self.params = params
+        h = C.c_void_p()
+        st = self.lib.gell_create(C.byref(params), stream, C.byref(h))
+        if st != 0:
+            msg = self.lib.gell_last_error(None).decode()
+            raise GellError(f"gell_create: {msg} ({st})")
+        self.h = h.value
+        self.N = params.N
+        self.V = params.N ** 3
+
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            raise GellError(f"{what}: {self.lib.gell_last_error(self.h).decode()} ({st})")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.gell_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- the five operations -------------------------------------------------
+    def fm_update(self, cells: DeviceBuffer | int, n: int) -> None:
+        self._check(self.lib.gell_fm_update(self.h, _ptr(cells), n), "gell_fm_update")
+
+    def prelod(self, s: int, cells: DeviceBuffer | int, n: int) -> None:
+        self._check(self.lib.gell_prelod(self.h, s, _ptr(cells), n), "gell_prelod")
+
+    def lod_substep(self, s: int) -> None:
+        self._check(self.lib.gell_lod_substep(self.h, s), "gell_lod_substep")
+
+    def birth(self, cells: DeviceBuffer | int, n: int, step: int) -> int:
+        out = C.c_int32()
+        self._check(self.lib.gell_birth(self.h, _ptr(cells), n, step, C.byref(out)), "gell_birth")
+        return out.value
+
+    def death(self, cells: DeviceBuffer | int, n: int) -> int:
+        out = C.c_int32()
+        self._check(self.lib.gell_death(self.h, _ptr(cells), n, C.byref(out)), "gell_death")
+        return out.value
+
+    def sync(self) -> None:
+        self._check(self.lib.gell_sync(self.h), "gell_sync")
+
+    # -- state access ----------------------------------------------------------
+    def field_ptr(self, s: int) -> int:
+        return self.lib.gell_field(self.h, s)
+
+    def field(self, s: int) -> np.ndarray:
+        self.sync()
+        return download_ptr(self.field_ptr(s), np.float32, self.V)
+
+    def set_field(self, s: int, a: np.ndarray) -> None:
+        self.sync()
+        upload_ptr(self.field_ptr(s), np.asarray(a, np.float32).ravel())
+
+    def uptake_field(self, s: int) -> np.ndarray:
+        self.sync()
+        return download_ptr(self.lib.gell_uptake_field(self.h, s), np.float32, self.V)
+
+    def sorted_keys(self, n: int) -> np.ndarray:
+        self.sync()
+        return download_ptr(self.lib.gell_sorted_keys(self.h), np.int32, n)
+
+    def sorted_order(self, n: int) -> np.ndarray:
+        self.sync()
+        return download_ptr(self.lib.gell_sorted_order(self.h), np.int32, n)
+
+    def voxel_ranges(self) -> tuple[np.ndarray, np.ndarray]:
+        self.sync()
+        se = download_ptr(self.lib.gell_voxel_ranges(self.h), np.int32, 2 * self.V).reshape(-1, 2)
+        return se[:, 0].copy(), se[:, 1].copy()
+
+    def birth_slots(self, n: int) -> np.ndarray:
+        self.sync()
+        return download_ptr(self.lib.gell_birth_slots(self.h), np.int32, n)
+
+    # -- profiling -------------------------------------------------------------
+    def profile(self, enable: bool = True) -> None:
+        self._check(self.lib.gell_profile_enable(self.h, int(enable)), "gell_profile_enable")
+
+    def profile_read(self, reset: bool = True) -> dict:
+        ms = (C.c_double * N_PHASES)()
+        calls = (C.c_int64 * N_PHASES)()
+        self._check(self.lib.gell_profile_read(self.h, ms, calls, int(reset)), "gell_profile_read")
+        return {PHASE_NAMES[q]: (ms[q], calls[q]) for q in range(N_PHASES)}
+
+
+def _ptr(x) -> int:
+    return x.ptr if isinstance(x, DeviceBuffer) else int(x)
+
+
+def float_loop_count(end: float, step: float) -> int:
+    """Trip count of ``for (float t = 0; t < end; t += step)`` (Main.cu:108,114,121)."""
+    t = np.float32(0.0)
+    end, step = np.float32(end), np.float32(step)
+    c = 0
+    while t < end:
+        c += 1
+        t = np.float32(t + step)
+    return c
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    if load_library().gell_device_name(buf, 256) != 0:
+        raise GellError("no device")
+    return buf.value.decode()

@@ -1,0 +1,180 @@
+/*
+ * gell_abi.h -- C ABI of the MI355X per-step hot path (mechanics, diffusion,
+ * proliferation/death) of the Gell tumour-spheroid simulator.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no C++ types and
+ * no exceptions cross this boundary. Every entry point returns a gell_status
+ * (0 = ok, < 0 = error); the message of the last error on a context is read
+ * with gell_last_error().
+ *
+ * Threading: one context is driven by one host thread; all work is enqueued
+ * on the context's HIP stream. Entry points that return a host-side count
+ * (gell_birth, gell_death) synchronise that stream; the others are
+ * asynchronous. Cell arrays are caller-owned device memory; all scratch is
+ * context-owned.
+ *
+ * Reference interfaces replaced (paths relative to the Gell sources):
+ *   gell_fm_update   <- MechanicsMesh_struct::FM_update    include/FMkernel.cuh:181-202
+ *   gell_prelod      <- Mesh_struct::PreLOD                include/LODkernel.cuh:270-278
+ *   gell_lod_substep <- Mesh_struct::LODsolver             include/LODkernel.cuh:280-300
+ *   gell_birth       <- CellBirth_kernel                   include/PDkernel.cuh:88-98
+ *   gell_death       <- CellDeath_kernel                   include/PDkernel.cuh:100-117
+ *   gell_create      <- MechanicsMesh_struct() + Mesh_struct(conc, D, decay)
+ *                       include/FMkernel.cuh:167-172, include/LODkernel.cuh:205-225,239-259
+ *   gell_field       <- public member Mesh_struct::p       include/LODkernel.cuh:199
+ */
+#ifndef GELL_ABI_H
+#define GELL_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GELL_ABI_VERSION 1
+#define GELL_MAX_SUBSTRATES 4
+
+typedef struct gell_f3 {
+    float x, y, z;
+} gell_f3;
+
+/*
+ * The 72-byte agent record, field for field the reference's `struct Cell`
+ * (include/Cells.cuh:9-47): bool sign @0 (3 pad bytes), int mesh_idx @4,
+ * float3 pos @8, force @20, oldforce @32, float r @44, V @48, Vf @52,
+ * Vcs @56, Vns @60, int phase @64, float cell_clock @68. sign is 0 or 1.
+ * Phases: 0 quiescent, 1 premitotic, 2 postmitotic, 3 apoptotic,
+ * 4 early necrotic, 5 late necrotic (include/Cells.cuh:25-30).
+ */
+typedef struct gell_cell {
+    uint8_t sign;
+    uint8_t pad_[3];
+    int32_t mesh_idx;
+    gell_f3 pos;
+    gell_f3 force;
+    gell_f3 oldforce;
+    float r;
+    float V;
+    float Vf;
+    float Vcs;
+    float Vns;
+    int32_t phase;
+    float cell_clock;
+} gell_cell;
+
+/*
+ * One diffusing substrate. The reference hard-codes oxygen
+ * (include/Definitions.h:21-25); substrate 0 is the one the phase model
+ * reads (PDkernel.cuh:37-39).
+ */
+typedef struct gell_substrate {
+    float boundary;    /* Dirichlet value on the two outer layers (O2_Default_Concentration, 38 mmHg) */
+    float D;           /* diffusion coefficient, um^2/min (O2_Diffusion_coef, 1e5) */
+    float decay;       /* linear decay, 1/min (O2_Decay_rate, 0.1) */
+    float uptake;      /* per-cell uptake per unit volume fraction, 1/min (Tumor_O2_consumption, -10) */
+    float dead_factor; /* uptake factor for phases >= 3 (Dead_O2_consumption_factor, 0.1) */
+    float initial;     /* initial field value (the reference starts at the boundary value) */
+} gell_substrate;
+
+/*
+ * Runtime mirror of include/Definitions.h:7-47. The mechanics bins are the
+ * diffusion voxels, as in the reference (FMkernel.cuh:25, Voxel_length).
+ */
+typedef struct gell_params {
+    int32_t N;            /* voxels per axis (Voxel_num); 2 < N <= 1024 */
+    float dx;             /* voxel edge, um (Voxel_length) */
+    int32_t capacity;     /* cell-array capacity (Max_Cell_num) */
+    int32_t n_substrates; /* 1..GELL_MAX_SUBSTRATES */
+    float dt_bio;         /* Biology_dt, min */
+    float dt_mech;        /* Mechanics_dt, min */
+    float dt_diff;        /* Diffusion_dt, min */
+    int32_t reserved_;
+    uint64_t seed;        /* key of the Philox4x32-10 stream used by birth */
+    gell_substrate sub[GELL_MAX_SUBSTRATES];
+} gell_params;
+
+typedef struct gell_ctx gell_ctx;
+
+typedef enum gell_status {
+    GELL_OK = 0,
+    GELL_E_ARG = -1,         /* bad argument (null pointer, size out of range) */
+    GELL_E_HIP = -2,         /* a HIP runtime call failed */
+    GELL_E_NOMEM = -3,       /* device allocation failed */
+    GELL_E_UNSUPPORTED = -4, /* parameter combination this build does not support */
+    GELL_E_NODEVICE = -5     /* no usable gfx950 device */
+} gell_status;
+
+/* Fill *p with the reference's Definitions.h values (oxygen only). */
+void gell_default_params(gell_params* p);
+
+/* Build the context: cell-list arrays, fields (set to sub[s].initial),
+ * uptake maps (zero), Thomas pre-factorisation, sort scratch. `stream` is a
+ * hipStream_t (NULL = the null stream). */
+int gell_create(const gell_params* params, void* stream, gell_ctx** out);
+void gell_destroy(gell_ctx* ctx);
+const char* gell_last_error(const gell_ctx* ctx);
+const char* gell_status_string(int status);
+
+/* Mechanics step on d_cells[0, n): Morton keys, stable sort of the records
+ * by key, per-voxel [s, e] ranges of live cells, adhesion/repulsion forces,
+ * Adams-Bashforth-2 movement. In place, asynchronous. */
+int gell_fm_update(gell_ctx* ctx, gell_cell* d_cells, int32_t n);
+
+/* Uptake map of one substrate from d_cells[0, n) (per-voxel sum of the
+ * live cells' uptake rates, taken at mesh_idx). Asynchronous. */
+int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int32_t n);
+
+/* One diffusion substep of one substrate: double-layer Dirichlet boundary,
+ * implicit uptake, LOD x/y/z Thomas sweeps. Asynchronous. */
+int gell_lod_substep(gell_ctx* ctx, int32_t substrate);
+
+/* Volume/phase update and stochastic division of d_cells[0, n) for biology
+ * step `bio_step`; daughters are appended at [n, *n_out). Synchronises. */
+int gell_birth(gell_ctx* ctx, gell_cell* d_cells, int32_t n, uint64_t bio_step, int32_t* n_out);
+
+/* Stable removal of records with sign == 0 from d_cells[0, n). Synchronises. */
+int gell_death(gell_ctx* ctx, gell_cell* d_cells, int32_t n, int32_t* n_out);
+
+/* Device pointers owned by the context (valid until gell_destroy). */
+float* gell_field(gell_ctx* ctx, int32_t substrate);        /* N^3 floats, x fastest */
+float* gell_uptake_field(gell_ctx* ctx, int32_t substrate); /* N^3 floats (p_rate) */
+
+/* Cell-list state of the last gell_fm_update, device pointers:
+ * sorted keys (Morton code, -1 outside; n entries), the permutation (sorted
+ * slot -> input slot; n entries), and the per-voxel live ranges as
+ * interleaved pairs (s[v], e[v]) for v < N^3 -- (capacity+1, -1) when the
+ * voxel holds no live cell (Mech_Mesh_s / Mech_Mesh_e, FMkernel.cuh:38-53). */
+const int32_t* gell_sorted_keys(gell_ctx* ctx);
+const int32_t* gell_sorted_order(gell_ctx* ctx);
+const int32_t* gell_voxel_ranges(gell_ctx* ctx);
+
+/* Birth events of the last gell_birth: per input slot, the daughter slot
+ * (>= n) or -1. Device pointer, capacity entries. */
+const int32_t* gell_birth_slots(gell_ctx* ctx);
+
+int gell_sync(gell_ctx* ctx);
+
+/* Per-phase device time, recorded with HIP events on the context stream
+ * while profiling is on. Phases: 0 fm_keys, 1 fm_sort, 2 fm_gather,
+ * 3 fm_force_move, 4 prelod, 5 lod, 6 birth, 7 death. ms_out and
+ * calls_out hold GELL_N_PHASES entries; reading synchronises. */
+#define GELL_N_PHASES 8
+int gell_profile_enable(gell_ctx* ctx, int enable);
+int gell_profile_read(gell_ctx* ctx, double* ms_out, int64_t* calls_out, int reset);
+
+/* Thin device-memory helpers for hosts without their own HIP binding. */
+void* gell_dev_alloc(size_t bytes);
+void gell_dev_free(void* p);
+int gell_copy_h2d(void* dst, const void* src, size_t bytes);
+int gell_copy_d2h(void* dst, const void* src, size_t bytes);
+int gell_copy_d2d(void* dst, const void* src, size_t bytes);
+int gell_device_sync(void);
+int gell_device_name(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GELL_ABI_H */

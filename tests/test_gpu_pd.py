@@ -1,0 +1,104 @@
+"""Uptake map, birth and death on the GPU against the oracle
+(PreLOD LODkernel.cuh:270-278; CellBirth_kernel / CellDeath_kernel
+PDkernel.cuh:27-117).
+
+Uptake maps are bit-exact (integer fixed-point sums). Birth events (which
+mothers divide, which slot each daughter gets, the new count) are bit-exact;
+records match exactly except the division geometry, whose sinf/cosf/acosf
+may differ by ulps between the device and host maths libraries."""
+import numpy as np
+import pytest
+
+from gell_amd import runtime as rt
+from helpers import GpuState, compare_cells, log, mixed_cells, params, rel_l2, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n_sub", [1, 2])
+def test_prelod_matches_oracle(n_sub, oracle):
+    N, n = 32, 8000
+    P = params(N, capacity=n, n_sub=n_sub)
+    cells = mixed_cells(N, 10.0, n, n, seed=21, dense=True)
+    st = GpuState(P, cells)
+    for step in range(3):
+        st.ctx.fm_update(st.buf, n)
+        for s in range(n_sub):
+            st.ctx.prelod(s, st.buf, n)
+        got_cells = st.cells()
+        for s in range(n_sub):
+            want = oracle.prelod(P, s, got_cells, n)
+            got = st.ctx.uptake_field(s)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"sub {s} step {step}"
+    log(f"[prelod] n_sub={n_sub} ok")
+    st.close()
+
+
+def _birth_state(N, n, capacity, seed):
+    cells = mixed_cells(N, 10.0, n, capacity, seed=seed, outside=0)
+    rng = np.random.default_rng(seed)
+    live = cells[:n]
+    # phase 1 near the end of its 780-min window divides this step
+    ph = rng.choice([0, 1, 1, 2, 3, 4, 5], n).astype(np.int32)
+    live["phase"] = ph
+    clk = np.where(ph == 1, 780.0 - 6.0 * rng.integers(0, 2, n), rng.random(n) * 150.0)
+    live["cell_clock"] = clk.astype(np.float32)
+    live["Vcs"] = np.where(rng.random(n) < 0.1, 20.0, live["Vcs"]).astype(np.float32)
+    p0 = (rng.random(N ** 3) * 40.0).astype(np.float32)
+    return cells, p0
+
+
+@pytest.mark.parametrize("capacity_room", [None, 37])
+def test_birth_matches_oracle(capacity_room, oracle):
+    N, n = 32, 5000
+    cap = 2 * n if capacity_room is None else n + capacity_room
+    P = params(N, capacity=cap, seed=987654321)
+    cells, p0 = _birth_state(N, n, cap, seed=8)
+    st = GpuState(P, cells)
+    st.ctx.set_field(0, p0)
+    n_gpu = st.ctx.birth(st.buf, n, 17)
+    got = st.cells()
+    slots = st.ctx.birth_slots(n)
+    want = cells.copy()
+    n_cpu, wslots = oracle.birth(P, p0, want, n, 17)
+    log(f"[birth] cap={cap} n {n} -> gpu {n_gpu} cpu {n_cpu}")
+    assert n_gpu == n_cpu
+    assert np.array_equal(slots, wslots)
+    assert n_cpu > n
+    compare_cells(got, want, n_cpu, exact=False, skip=("pos",))
+    # all non-geometry fields bit-exact
+    for f in ("r", "V", "Vf", "Vcs", "Vns", "cell_clock", "force", "oldforce"):
+        assert ulp_diff(got[f][:n_cpu], want[f][:n_cpu]) == 0, f
+    div = np.concatenate([np.nonzero(wslots >= 0)[0], wslots[wslots >= 0]])
+    same = np.setdiff1d(np.arange(n_cpu), div)
+    assert ulp_diff(got["pos"][same], want["pos"][same]) == 0
+    assert rel_l2(got["pos"][:n_cpu], want["pos"][:n_cpu]) <= 1e-6
+    st.close()
+
+
+def test_birth_full_capacity_does_nothing(oracle):
+    N, n = 16, 100
+    P = params(N, capacity=n)
+    cells, p0 = _birth_state(N, n, n, seed=9)
+    st = GpuState(P, cells)
+    st.ctx.set_field(0, p0)
+    assert st.ctx.birth(st.buf, n, 3) == n
+    got = st.cells()
+    assert np.array_equal(got.view(np.uint8), cells.view(np.uint8))
+    assert np.all(st.ctx.birth_slots(n) == -1)
+    st.close()
+
+
+@pytest.mark.parametrize("n", [0, 1, 777, 100_000])
+def test_death_matches_oracle(n, oracle):
+    N = 32
+    P = params(N, capacity=max(n, 1))
+    cells = mixed_cells(N, 10.0, n, max(n, 1), seed=n + 1, dead_frac=0.4, outside=0)
+    st = GpuState(P, cells)
+    got_n = st.ctx.death(st.buf, n)
+    got = st.cells()
+    want = cells.copy()
+    want_n = oracle.death(want, n)
+    assert got_n == want_n == int(cells["sign"][:n].sum())
+    assert np.array_equal(got[:got_n].view(np.uint8), want[:want_n].view(np.uint8))
+    st.close()
