@@ -1,0 +1,300 @@
+#!/usr/bin/env python3
+"""Throughput of the Gell per-step hot path on one MI355X.
+
+Workload (default, "L"): 256^3 voxels at 10 um, 1,000,000 cells seeded
+uniformly in a 1.2 mm-radius ball at the domain centre (radii 4.5-5.5 um),
+two substrates (oxygen and glucose). One timed step is one biology step of the
+reference's time loop (Main.cu:108-127):
+
+    birth -> 61 x ( mechanics step, uptake map per substrate,
+                    11 x LOD substep per substrate ) -> death
+
+with the trip counts the reference's float loops produce. Inputs are resident
+in HBM before the timed region; W untimed steps run first.
+
+Output: one JSON line on stdout with agent-steps/s (cells x mechanics steps
+per second) as `value`, voxel-updates/s (voxels x substrates x LOD substeps
+per second) beside it, the roofline of the dominant kernel (the LOD substep,
+timed live with HIP events on the library's stream) and the CPU oracle timed
+on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "agent-steps/s & voxel-updates/s, 1M cells/256^3; surrogate vs LOD at 1e-3 L2"
+HBM_PEAK_GBS = 8000.0
+LOD_BYTES_PER_VOXEL = 28  # x sweep 12 (p, p_rate in, p out) + y 8 + z 8
+
+# Substrate constants: oxygen as in the reference's Definitions.h:21-25;
+# glucose with a smaller diffusivity, no decay and its own boundary value.
+SUBSTRATES = [
+    dict(boundary=38.0, D=1.0e5, decay=0.1, uptake=-10.0, dead_factor=0.1, initial=38.0),
+    dict(boundary=5.0, D=4.0e4, decay=0.0, uptake=-1.0, dead_factor=0.1, initial=5.0),
+]
+
+CONFIGS = {
+    "L": dict(N=256, dx=10.0, cells=1_000_000, radius=1200.0, n_sub=2, capacity=2_000_000),
+    "S": dict(N=64, dx=10.0, cells=10_000, radius=250.0, n_sub=1, capacity=20_000),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(cfg: dict, seed: int):
+    from gell_amd import runtime as rt
+    from gell_amd import spheroid
+
+    P = rt.make_params(N=cfg["N"], dx=cfg["dx"], capacity=cfg["capacity"], seed=seed,
+                       substrates=SUBSTRATES[: cfg["n_sub"]])
+    cells = spheroid.make_cells(cfg["cells"], cfg["N"], cfg["dx"], cfg["radius"], seed, capacity=cfg["capacity"])
+    return P, cells
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def gpu_leg(cfg, P, cells, steps, warmup):
+    from gell_amd import runtime as rt
+
+    n = cfg["cells"]
+    ctx = rt.Context(P)
+    buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
+    buf.upload(cells)
+    mech = rt.float_loop_count(P.dt_bio, P.dt_mech)
+    diff = rt.float_loop_count(P.dt_mech, P.dt_diff)
+
+    def bio_step(b, n):
+        n = ctx.birth(buf, n, b)
+        agents = 0
+        for _ in range(mech):
+            ctx.fm_update(buf, n)
+            for s in range(P.n_substrates):
+                ctx.prelod(s, buf, n)
+            for _ in range(diff):
+                for s in range(P.n_substrates):
+                    ctx.lod_substep(s)
+            agents += n
+        n = ctx.death(buf, n)
+        return n, agents
+
+    for w in range(warmup):
+        n, _ = bio_step(w, n)
+        log(f"[bench] warmup step {w} done, n={n}")
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    agents = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        n, a = bio_step(warmup + k, n)
+        agents += a
+    ctx.sync()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    log(f"[bench] timed {steps} steps in {t1 - t0:.3f} s, n={n}")
+    out = dict(seconds=t1 - t0, agents=agents, n_final=n, mech=mech, diff=diff, prof=prof,
+               device=rt.device_name())
+    buf.free()
+    ctx.close()
+    return out
+
+
+def cpu_leg(cfg, P, cells, budget_s):
+    """The OpenMP oracle on this host's cores: whole mechanics steps (FM,
+    uptake maps, 11 LOD substeps per substrate) of the same workload, until
+    about budget_s seconds have been spent."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    orc = pyoracle.Oracle(omp=True)
+    n = cfg["cells"]
+    V = cfg["N"] ** 3
+    fields = np.concatenate([np.full(V, P.sub[s].initial, np.float32) for s in range(P.n_substrates)])
+    work = cells.copy()
+    times = np.zeros(3, np.float64)
+    t0 = time.perf_counter()
+    orc.run(P, work, n, fields, 1, mech_limit=1, with_pd=False, times=times)
+    t1 = time.perf_counter()
+    per = t1 - t0
+    more = max(0, int(budget_s / max(per, 1e-3)) - 1)
+    if more:
+        orc.run(P, work, n, fields, 1, mech_limit=more, with_pd=False, times=times)
+    t2 = time.perf_counter()
+    steps = 1 + more
+    secs = t2 - t0
+    from gell_amd import runtime as rt
+
+    diff = rt.float_loop_count(P.dt_mech, P.dt_diff)
+    return dict(
+        value=n * steps / secs,
+        voxel_updates_per_s=V * P.n_substrates * diff * steps / secs,
+        seconds=secs,
+        steps=steps,
+        threads=orc.threads(),
+        phase_ms=dict(fm=times[0], prelod=times[1], lod=times[2]),
+    )
+
+
+def only_leg(cfg, P, cells, what: str, steps: int, warmup: int) -> dict:
+    """One kernel group alone, for profiler runs: `lod` (LOD substeps of
+    substrate 0 after one mechanics step + uptake map), `fm` (mechanics steps)
+    or `copy` (the streaming copy kernel over 256 MiB, the byte-counter
+    calibration)."""
+    from gell_amd import runtime as rt
+
+    n = cfg["cells"]
+    ctx = rt.Context(P)
+    buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
+    buf.upload(cells)
+    ctx.fm_update(buf, n)
+    for s in range(P.n_substrates):
+        ctx.prelod(s, buf, n)
+    ctx.sync()
+    out = {"what": what, "steps": steps}
+    if what == "copy":
+        nbytes = 256 << 20
+        a, b = rt.DeviceBuffer(nbytes), rt.DeviceBuffer(nbytes)
+        for _ in range(warmup + steps):
+            ctx.lib.gell_stream_copy(b.ptr, a.ptr, nbytes, None)
+        ctx.lib.gell_device_sync()
+        out["bytes_per_launch_each_way"] = nbytes
+        a.free()
+        b.free()
+    else:
+        for k in range(warmup + steps):
+            if k == warmup:
+                ctx.sync()
+                ctx.profile(True)
+                ctx.profile_read(reset=True)
+            if what == "lod":
+                ctx.lod_substep(0)
+            else:
+                ctx.fm_update(buf, n)
+        prof = ctx.profile_read(reset=True)
+        out["phase_us"] = {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items() if v[1]}
+    buf.free()
+    ctx.close()
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=2, help="timed biology steps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed biology steps")
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="L")
+    ap.add_argument("--seed", type=int, default=20240611)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
+    ap.add_argument("--only", choices=["lod", "fm", "copy"], default=None,
+                    help="run one kernel group alone (profiler runs); prints its timings")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    cfg = CONFIGS[args.config]
+    log(f"[bench] building workload {args.config}")
+    P, cells = make_workload(cfg, args.seed)
+    if args.only:
+        print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup)), flush=True)
+        return 0
+    g = gpu_leg(cfg, P, cells, args.steps, args.warmup)
+
+    secs = g["seconds"]
+    V = cfg["N"] ** 3
+    lod_subs = args.steps * g["mech"] * g["diff"] * P.n_substrates
+    lod_ms, lod_calls = g["prof"]["lod"]
+    lod_us = lod_ms / max(lod_calls, 1) * 1e3
+    achieved = LOD_BYTES_PER_VOXEL * V / (lod_us * 1e-6) / 1e9
+    traffic = read_traffic(f"lod_substep_{cfg['N']}_traffic.txt")
+    phases = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in g["prof"].items() if v[1]}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("[bench] CPU oracle leg")
+        c = cpu_leg(cfg, P, cells, args.cpu_seconds)
+        cpu = {
+            "value": c["value"],
+            "unit": "agent-steps/s",
+            "cores": c["threads"],
+            "kind": "port",
+            "sample": (f"{c['steps']} mechanics steps (FM + uptake maps + {g['diff']} LOD substeps x "
+                       f"{P.n_substrates} substrates, no PD) of the same workload, OpenMP oracle, "
+                       f"{c['seconds']:.1f} s"),
+            "voxel_updates_per_s": c["voxel_updates_per_s"],
+            "host_nproc": os.cpu_count(),
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": g["agents"] / secs,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded spheroid)",
+        "config": {
+            "workload": (f"{args.config}: {cfg['N']}^3 voxels at {cfg['dx']:g} um, {cfg['cells']} cells in a "
+                         f"{cfg['radius']:g} um ball, {P.n_substrates} substrate(s); step = birth + "
+                         f"{g['mech']} x (FM + uptake maps + {g['diff']} LOD substeps per substrate) + death"),
+            "grid": cfg["N"],
+            "cells": cfg["cells"],
+            "substrates": P.n_substrates,
+            "mech_steps_per_step": g["mech"],
+            "lod_substeps_per_mech_step": g["diff"],
+        },
+        "voxel_updates_per_s": V * lod_subs / secs,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "LOD substep (x sweep with boundary + uptake, y sweep, z sweep)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": LOD_BYTES_PER_VOXEL * V,
+            "us_per_launch": lod_us,
+        },
+        "cpu_baseline": cpu,
+        "phase_us": phases,
+        "device": g["device"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -42,21 +42,39 @@ int check_cells(gell_ctx* ctx, const void* cells, int32_t n) {
     return GELL_OK;
 }
 
+hipEvent_t take_event(gell_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one phase with a pair of events on the context stream while
+// profiling is on.
 struct PhaseScope {
     gell_ctx* ctx;
     int phase;
     hipEvent_t a = nullptr;
     PhaseScope(gell_ctx* c, int ph) : ctx(c), phase(ph) {
-        if (ctx->prof && hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, ctx->stream);
+        if (!ctx->prof) return;
+        a = take_event(ctx);
+        if (a && hipEventRecord(a, ctx->stream) != hipSuccess) {
+            ctx->event_pool.push_back(a);
+            a = nullptr;
+        }
     }
     ~PhaseScope() {
         if (!a) return;
-        hipEvent_t b = nullptr;
-        if (hipEventCreate(&b) == hipSuccess && hipEventRecord(b, ctx->stream) == hipSuccess) {
+        hipEvent_t b = take_event(ctx);
+        if (b && hipEventRecord(b, ctx->stream) == hipSuccess) {
             ctx->marks.push_back({phase, a, b});
         } else {
-            (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
+            ctx->event_pool.push_back(a);
+            if (b) ctx->event_pool.push_back(b);
         }
     }
 };
@@ -68,6 +86,8 @@ void release(gell_ctx* c) {
         (void)hipEventDestroy(m.b);
     }
     c->marks.clear();
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    c->event_pool.clear();
     for (int q = 0; q < 2; ++q) {
         dfree(c->keys[q]);
         dfree(c->vals[q]);
@@ -319,6 +339,19 @@ int gell_sync(gell_ctx* ctx) {
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "sync");
 }
 
+int gell_tune(gell_ctx* ctx, int knob, int value) {
+    if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
+    if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
+    if (value < 0 || value > 1) return fail(ctx, GELL_E_ARG, "knob value out of range");
+    ctx->tune[knob] = value;
+    return GELL_OK;
+}
+
+int gell_tune_get(const gell_ctx* ctx, int knob) {
+    if (!ctx || knob < 0 || knob >= GELL_N_KNOBS) return -1;
+    return ctx->tune[knob];
+}
+
 int gell_profile_enable(gell_ctx* ctx, int enable) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     ctx->prof = enable != 0;
@@ -335,8 +368,8 @@ int gell_profile_read(gell_ctx* ctx, double* ms_out, int64_t* calls_out, int res
             ctx->prof_ms[m.phase] += ms;
             ctx->prof_calls[m.phase] += 1;
         }
-        (void)hipEventDestroy(m.a);
-        (void)hipEventDestroy(m.b);
+        ctx->event_pool.push_back(m.a);
+        ctx->event_pool.push_back(m.b);
     }
     ctx->marks.clear();
     for (int q = 0; q < GELL_N_PHASES; ++q) {

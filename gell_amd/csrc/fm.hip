@@ -304,6 +304,59 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
 // ---------------------------------------------------------------------------
 // Force + movement
 // ---------------------------------------------------------------------------
+// Force on live cell i (forcekernel, FMkernel.cuh:56-118). With PREFETCH the
+// 27 stencil ranges are loaded before the pair loop, so their latencies
+// overlap; the visiting order (x outer, z inner, ascending slot) is the same.
+template <bool PREFETCH>
+__device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const int2* __restrict__ se, int32_t N,
+                                           float dx, float p1x, float p1y, float p1z, float r1, float& fx, float& fy,
+                                           float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    int2 rr[27];
+    if (PREFETCH) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+            rr[q] = inside(xi, yi, zi, N) ? se[lin(xi, yi, zi, N)] : make_int2(0, -1);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+        int2 r;
+        if (PREFETCH) {
+            r = rr[q];
+        } else {
+            if (!inside(xi, yi, zi, N)) continue;
+            r = se[lin(xi, yi, zi, N)];
+        }
+        if (r.y < 0) continue;
+        for (int j = r.x; j <= r.y; ++j) {
+            const float4 o = nb[j];
+            if (__float_as_uint(o.w) == kDeadTag) continue;
+            // FMkernel.cuh:89-109
+            const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
+            const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+            const float distance = sqrtf((float)((double)ss + 0.0001));
+            if (distance >= thres) continue;
+            const float rsum = o.w + r1;
+            const float R = rsum > 0.01f ? rsum : 0.01f;
+            thres = (float)((double)R * 1.25);
+            if (distance >= thres) continue;
+            const float qr = 1.f - distance / R;
+            const float qa = 1.f - distance / thres;
+            const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
+            const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
+            const float cc = coefr - coefa;
+            fx += ddx * cc;
+            fy += ddy * cc;
+            fz += ddz * cc;
+        }
+    }
+}
+
+template <bool PREFETCH>
 __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
                                                         const float4* __restrict__ nb, const int2* __restrict__ se,
                                                         gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
@@ -313,38 +366,8 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
     CellRegs c = load_cell(sorted, i);
     if (csign(c)) {
         const float p1x = cf(c, W_PX), p1y = cf(c, W_PY), p1z = cf(c, W_PZ);
-        const float r1 = cf(c, W_R);
-        const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
-        for (int xi = px - 1; xi <= px + 1; ++xi)
-            for (int yi = py - 1; yi <= py + 1; ++yi)
-                for (int zi = pz - 1; zi <= pz + 1; ++zi) {
-                    if (!inside(xi, yi, zi, N)) continue;
-                    const int2 r = se[lin(xi, yi, zi, N)];
-                    if (r.y < 0) continue;
-                    for (int j = r.x; j <= r.y; ++j) {
-                        const float4 q = nb[j];
-                        if (__float_as_uint(q.w) == kDeadTag) continue;
-                        // FMkernel.cuh:89-109
-                        const float ddx = p1x - q.x, ddy = p1y - q.y, ddz = p1z - q.z;
-                        const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
-                        const float distance = sqrtf((float)((double)ss + 0.0001));
-                        if (distance >= thres) continue;
-                        const float rsum = q.w + r1;
-                        const float R = rsum > 0.01f ? rsum : 0.01f;
-                        thres = (float)((double)R * 1.25);
-                        if (distance >= thres) continue;
-                        const float qr = 1.f - distance / R;
-                        const float qa = 1.f - distance / thres;
-                        const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
-                        const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
-                        const float cc = coefr - coefa;
-                        fx += ddx * cc;
-                        fy += ddy * cc;
-                        fz += ddz * cc;
-                    }
-                }
+        cell_force<PREFETCH>(nb, se, N, dx, p1x, p1y, p1z, cf(c, W_R), fx, fy, fz);
         // movementkernel, FMkernel.cuh:130-158
         const float h = dt / 2.f;
         const float ox = cf(c, W_OX), oy = cf(c, W_OY), oz = cf(c, W_OZ);
@@ -434,8 +457,13 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fm_force_move, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->sorted,
-                       ctx->nb, ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    const dim3 grid((n + kBlock - 1) / kBlock);
+    if (ctx->tune[GELL_KNOB_FM_PREFETCH])
+        hipLaunchKernelGGL(fm_force_move<true>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    else
+        hipLaunchKernelGGL(fm_force_move<false>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
     return hipGetLastError();
 }
 

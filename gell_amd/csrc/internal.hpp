@@ -50,6 +50,9 @@ struct gell_ctx {
     gell_cell* compact = nullptr;  // death scratch
     int64_t blk_cap = 0;
 
+    // Kernel-variant knobs (GELL_KNOB_*).
+    int tune[GELL_N_KNOBS] = {1, 1};
+
     // Profiling.
     bool prof = false;
     struct Mark {
@@ -57,6 +60,7 @@ struct gell_ctx {
         hipEvent_t a, b;
     };
     std::vector<Mark> marks;
+    std::vector<hipEvent_t> event_pool;  // recycled events
     double prof_ms[GELL_N_PHASES] = {};
     int64_t prof_calls[GELL_N_PHASES] = {};
 };

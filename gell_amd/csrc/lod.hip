@@ -123,12 +123,17 @@ __global__ __launch_bounds__(64) void lod_x_reg(float* __restrict__ p, const flo
 // DIR 1: y lines (i, k), DIR 2: z lines (i, j); lane = flat line index.
 // Raw buffer accesses keep one per-lane byte offset for the whole line and
 // put the step t * stride in the scalar offset, so no per-element addresses
-// are held in registers.
-template <int N, int DIR>
-__global__ __launch_bounds__(64) void lod_yz_reg(float* __restrict__ p, const float* __restrict__ E,
+// are held in registers. With SPILL > 0 the forward-sweep values Y[1..SPILL)
+// live in LDS ([t][lane], conflict-free) instead of registers, which lets two
+// waves share a SIMD; the arithmetic is the same either way.
+template <int N, int DIR, int SPILL>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPILL > 0 ? 2 : 1))) void lod_yz_reg(float* __restrict__ p, const float* __restrict__ E,
                                                  const float* __restrict__ F, float side, float bnd) {
     constexpr int64_t NN = (int64_t)N * N;
-    const int64_t L = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    constexpr int SP = SPILL > 0 ? SPILL : 1;
+    __shared__ float ys[SP][64];
+    const int lane = threadIdx.x;
+    const int64_t L = (int64_t)blockIdx.x * 64 + lane;
     const bool active = L < NN;
     const int64_t Lc = active ? L : 0;
     const int64_t base = (DIR == 1) ? (Lc % N) + NN * (Lc / N) : Lc;
@@ -137,11 +142,28 @@ __global__ __launch_bounds__(64) void lod_yz_reg(float* __restrict__ p, const fl
     const int voff = (int)(base * 4);
     float v[N];
 #pragma unroll
-    for (int t = 1; t < N - 1; ++t) v[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, t * stride_b, 0);
-    thomas_regs<N>(v, E, F, side, bnd);
-    if (!active) return;
+    for (int t = 1; t < N - 1; ++t)
+        v[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, t * stride_b, 0));
+    // Forward sweep (p[N-1] = bnd is not needed: Y[N-1] is never used).
+    float y = bnd;
 #pragma unroll
-    for (int t = 0; t < N; ++t) __builtin_amdgcn_raw_buffer_store_b32(v[t], rs, voff, t * stride_b, 0);
+    for (int t = 1; t < N - 1; ++t) {
+        y = v[t] - E[t] * y;
+        if (t < SPILL)
+            ys[t][lane] = y;
+        else
+            v[t] = y;
+    }
+    if (!active) return;
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, bnd), rs, voff, (N - 1) * stride_b, 0);
+    float nxt = bnd;
+#pragma unroll
+    for (int t = N - 2; t > 0; --t) {
+        const float yt = (t < SPILL) ? ys[t][lane] : v[t];
+        nxt = (yt - side * nxt) / F[t];
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, nxt), rs, voff, t * stride_b, 0);
+    }
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, bnd), rs, voff, 0, 0);
 }
 
 // Any N, any direction (dir 0 = x with fused boundary + uptake).
@@ -192,12 +214,18 @@ __global__ __launch_bounds__(256) void lod_sweep_any(float* __restrict__ p, cons
 
 template <int N>
 hipError_t launch_reg(float* p, const float* rate, const float* E, const float* F, float side, float bnd, float dt,
-                      hipStream_t st) {
+                      int spill, hipStream_t st) {
     const int64_t lines = (int64_t)N * N;
     const unsigned blocks = (unsigned)((lines + 63) / 64);
     hipLaunchKernelGGL((lod_x_reg<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
-    hipLaunchKernelGGL((lod_yz_reg<N, 1>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
-    hipLaunchKernelGGL((lod_yz_reg<N, 2>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    if (spill > 0 && N > 64) {
+        constexpr int S = N > 64 ? 64 : 1;
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, S>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, S>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    } else {
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    }
     return hipGetLastError();
 }
 
@@ -211,13 +239,14 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     const float* F = ctx->F[s];
     const float side = ctx->side[s], bnd = ctx->P.sub[s].boundary, dt = ctx->P.dt_diff;
     hipStream_t st = ctx->stream;
+    const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
     switch (N) {
-        case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, st);
-        case 64: return launch_reg<64>(p, rate, E, F, side, bnd, dt, st);
-        case 96: return launch_reg<96>(p, rate, E, F, side, bnd, dt, st);
-        case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, st);
-        case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, st);
-        case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, st);
+        case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 64: return launch_reg<64>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 96: return launch_reg<96>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, spill, st);
         default: break;
     }
     const int64_t lines = (int64_t)N * N;

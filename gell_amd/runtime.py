@@ -109,6 +109,8 @@ def load_library() -> C.CDLL:
         "gell_birth_slots": (vp, [vp]),
         "gell_sync": (C.c_int, [vp]),
         "gell_profile_enable": (C.c_int, [vp, C.c_int]),
+        "gell_tune": (C.c_int, [vp, C.c_int, C.c_int]),
+        "gell_tune_get": (C.c_int, [vp, C.c_int]),
         "gell_profile_read": (C.c_int, [vp, P(C.c_double), P(i64), C.c_int]),
         "gell_dev_alloc": (vp, [sz]),
         "gell_dev_free": (None, [vp]),
@@ -116,6 +118,7 @@ def load_library() -> C.CDLL:
         "gell_copy_d2h": (C.c_int, [vp, vp, sz]),
         "gell_copy_d2d": (C.c_int, [vp, vp, sz]),
         "gell_device_sync": (C.c_int, []),
+        "gell_stream_copy": (C.c_int, [vp, vp, sz, vp]),
         "gell_device_name": (C.c_int, [C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -313,6 +316,15 @@ class Context:
     def birth_slots(self, n: int) -> np.ndarray:
         self.sync()
         return download_ptr(self.lib.gell_birth_slots(self.h), np.int32, n)
+
+    # -- kernel variants -------------------------------------------------------
+    KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1}
+
+    def tune(self, knob: str, value: int) -> None:
+        self._check(self.lib.gell_tune(self.h, self.KNOBS[knob], int(value)), "gell_tune")
+
+    def tune_get(self, knob: str) -> int:
+        return self.lib.gell_tune_get(self.h, self.KNOBS[knob])
 
     # -- profiling -------------------------------------------------------------
     def profile(self, enable: bool = True) -> None:

@@ -156,6 +156,17 @@ const int32_t* gell_birth_slots(gell_ctx* ctx);
 
 int gell_sync(gell_ctx* ctx);
 
+/* Kernel-variant knobs (the autotuner's search space). Every value of every
+ * knob gives bit-identical results; only speed differs.
+ *   GELL_KNOB_LOD_YZ_LDS   0 = y/z sweep lines fully in registers,
+ *                          1 = first 64 forward values per line in LDS
+ *                              (two waves per SIMD), for N > 64
+ *   GELL_KNOB_FM_PREFETCH  0 = one stencil-range load per voxel in turn,
+ *                          1 = all 27 stencil ranges loaded up front */
+enum { GELL_KNOB_LOD_YZ_LDS = 0, GELL_KNOB_FM_PREFETCH = 1, GELL_N_KNOBS = 2 };
+int gell_tune(gell_ctx* ctx, int knob, int value);
+int gell_tune_get(const gell_ctx* ctx, int knob);
+
 /* Per-phase device time, recorded with HIP events on the context stream
  * while profiling is on. Phases: 0 fm_keys, 1 fm_sort, 2 fm_gather,
  * 3 fm_force_move, 4 prelod, 5 lod, 6 birth, 7 death. ms_out and
@@ -171,6 +182,10 @@ int gell_copy_h2d(void* dst, const void* src, size_t bytes);
 int gell_copy_d2h(void* dst, const void* src, size_t bytes);
 int gell_copy_d2d(void* dst, const void* src, size_t bytes);
 int gell_device_sync(void);
+/* Device-to-device copy by the library's streaming kernel (16-byte accesses)
+ * on `stream` (a hipStream_t, NULL = null stream); asynchronous. It is also
+ * the known-size kernel the HBM byte counters are calibrated against. */
+int gell_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 int gell_device_name(char* buf, int len);
 
 #ifdef __cplusplus

@@ -1,0 +1,52 @@
+"""Every kernel variant the autotuner may pick gives bit-identical results."""
+import numpy as np
+import pytest
+
+from gell_amd import runtime as rt
+from helpers import GpuState, mixed_cells, params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N", [64, 128, 192])
+def test_lod_variants_identical(N):
+    P = params(N)
+    rng = np.random.default_rng(N)
+    p = (20 + 18 * rng.random(N ** 3)).astype(np.float32)
+    rate = (-2 * rng.random(N ** 3)).astype(np.float32)
+    outs = []
+    for v in (0, 1):
+        st = GpuState(P, np.zeros(0, rt.CELL_DTYPE))
+        st.ctx.tune("lod_yz_lds", v)
+        assert st.ctx.tune_get("lod_yz_lds") == v
+        st.ctx.set_field(0, p)
+        rt.upload_ptr(st.ctx.lib.gell_uptake_field(st.ctx.h, 0), rate)
+        for _ in range(2):
+            st.ctx.lod_substep(0)
+        outs.append(st.ctx.field(0))
+        st.close()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_fm_variants_identical():
+    N, n = 64, 20000
+    P = params(N, capacity=n)
+    cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
+    outs = []
+    for v in (0, 1):
+        st = GpuState(P, cells)
+        st.ctx.tune("fm_prefetch", v)
+        for _ in range(3):
+            st.ctx.fm_update(st.buf, n)
+        outs.append(st.cells())
+        st.close()
+    assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
+
+
+def test_bad_knob_is_an_error():
+    st = GpuState(params(16), np.zeros(0, rt.CELL_DTYPE))
+    with pytest.raises(rt.GellError):
+        st.ctx.tune("lod_yz_lds", 7)
+    with pytest.raises(KeyError):
+        st.ctx.tune("nope", 1)
+    st.close()
