@@ -105,6 +105,9 @@ void release(gell_ctx* c) {
         dfree(c->rate[s]);
         dfree(c->E[s]);
         dfree(c->F[s]);
+        dfree(c->ptab[s][0]);
+        dfree(c->ptab[s][1]);
+        dfree(c->ptab[s][2]);
         dfree(c->acc[s]);
         dfree(c->rate_list[s]);
     }
@@ -205,6 +208,9 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         A(dalloc(&c->rate[s], (size_t)c->V));
         A(dalloc(&c->E[s], (size_t)P.N));
         A(dalloc(&c->F[s], (size_t)P.N));
+        A(dalloc(&c->ptab[s][0], (size_t)(4 * P.N + 16)));
+        A(dalloc(&c->ptab[s][1], (size_t)(4 * P.N + 64)));
+        A(dalloc(&c->ptab[s][2], (size_t)(4 * P.N + 256)));
         A(dalloc(&c->acc[s], (size_t)c->V));
         A(dalloc(&c->rate_list[s], cap));
     }
@@ -342,9 +348,17 @@ int gell_sync(gell_ctx* ctx) {
 int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
-    if (value < 0 || value > 1) return fail(ctx, GELL_E_ARG, "knob value out of range");
+    const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
+    const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8) : (value == 0 || value == 1);
+    if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     ctx->tune[knob] = value;
     return GELL_OK;
+}
+
+int gell_autotune(gell_ctx* ctx, int flags, double* lod_us_out) {
+    if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
+    hipError_t e = gell::lod_autotune(ctx, (flags & GELL_TUNE_BIT_EXACT) != 0, lod_us_out);
+    return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "autotune");
 }
 
 int gell_tune_get(const gell_ctx* ctx, int knob) {

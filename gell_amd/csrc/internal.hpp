@@ -37,6 +37,7 @@ struct gell_ctx {
     float* E[GELL_MAX_SUBSTRATES] = {};
     float* F[GELL_MAX_SUBSTRATES] = {};
     float side[GELL_MAX_SUBSTRATES] = {};
+    float* ptab[GELL_MAX_SUBSTRATES][3] = {};  // partitioned-solve tables, P = 2, 4, 8
     long long* acc[GELL_MAX_SUBSTRATES] = {};
     int32_t* rate_list[GELL_MAX_SUBSTRATES] = {};
     int32_t rate_list_n[GELL_MAX_SUBSTRATES] = {};
@@ -51,7 +52,7 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 1};
+    int tune[GELL_N_KNOBS] = {1, 1, 4, 4};
 
     // Profiling.
     bool prof = false;
@@ -72,6 +73,7 @@ enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, P
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s);
 hipError_t lod_init_coeffs(gell_ctx* ctx, int s);
+hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us);
 
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
 hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n);

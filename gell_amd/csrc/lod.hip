@@ -20,6 +20,8 @@
 //                  wave access; the whole line stays in registers.
 //   lod_sweep_any  any N: one lane per line, Y kept in place in the field.
 // Bytes per voxel-update (algorithmic): x 12 (p, p_rate in; p out), y 8, z 8.
+#include <cmath>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -110,11 +112,11 @@ __global__ __launch_bounds__(64) void lod_x_reg(float* __restrict__ p, const flo
 #pragma unroll
         for (int it = 0; it < Q; ++it) {
             const int l = it * LPI + lq;
-            const int so = sbase + (it * LPI * N + c * kChunk) * 4;
             const float* src = &tile[l * kTileLd + part * 4];
-            const float4 o = make_float4(src[0], src[1], src[2], src[3]);
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, o),
-                                                   rp, voff, so, 0);
+            // 16-byte stores go through global (FLAT) stores: the compiler's
+            // hazard model covers a VALU overwrite of their data registers.
+            *reinterpret_cast<float4*>(p + (L0 + l) * N + c * kChunk + part * 4) =
+                make_float4(src[0], src[1], src[2], src[3]);
         }
         __syncthreads();
     }
@@ -164,6 +166,164 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPILL > 0 ? 
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, nxt), rs, voff, t * stride_b, 0);
     }
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, bnd), rs, voff, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned line solve (P segments per line, one wave per segment).
+//
+// The line system of the reference's sweep is A x = d with
+//   row 0      [corner, side]            d = b  (the injected Y[0] = b)
+//   rows 1..   [side, diag, side]        d = p
+//   row N-1    [0 ... 0, 1]              d = b  (p[N-1] = b)
+// and x[0] overwritten with b afterwards. Splitting the rows into P blocks of
+// m = N / P, each block is solved alone with its own Thomas factors (El, Fl),
+// giving y_k; the exact solution is x_k = y_k - u_{k-1} W_k - t_{k+1} V_k,
+// where W_k, V_k are the block's spikes (its solution for the coupling
+// coefficient placed on its first / last row) and t_k, u_k are the first and
+// last unknowns of block k, obtained from the 2P boundary values of y through
+// the inverse of the reduced 2P x 2P system. Spikes and the reduced inverse
+// depend only on the matrix, which every line shares, so they are computed
+// once on the host (in double) and read as wave-uniform scalars.
+// Table layout (floats): El[N] | Fl[N] | W[N] | V[N] | Rinv[(2P)^2].
+// ---------------------------------------------------------------------------
+template <int N, int P>
+__device__ __forceinline__ void part_solve(float (&v)[N / P], const float* __restrict__ tab, int k, float side,
+                                           float bnd, float* __restrict__ bt, float* __restrict__ bb, int lane) {
+    constexpr int m = N / P;
+    const float* El = tab + k * m;
+    const float* Fl = tab + N + k * m;
+    const float* W = tab + 2 * N + k * m;
+    const float* Vs = tab + 3 * N + k * m;
+    const float* Rinv = tab + 4 * N;
+    // local forward / backward sweep
+#pragma unroll
+    for (int i = 1; i < m; ++i) v[i] = v[i] - El[i] * v[i - 1];
+    v[m - 1] = v[m - 1] / Fl[m - 1];
+#pragma unroll
+    for (int i = m - 2; i >= 0; --i) v[i] = (v[i] - side * v[i + 1]) / Fl[i];
+    // exchange block boundary values through LDS
+    bt[k * 64 + lane] = v[0];
+    bb[k * 64 + lane] = v[m - 1];
+    __syncthreads();
+    float um1 = 0.f, tp1 = 0.f;
+    if (k > 0) {
+        const float* row = Rinv + (2 * (k - 1) + 1) * (2 * P);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            um1 += row[2 * q] * bt[q * 64 + lane];
+            um1 += row[2 * q + 1] * bb[q * 64 + lane];
+        }
+    }
+    if (k < P - 1) {
+        const float* row = Rinv + (2 * (k + 1)) * (2 * P);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            tp1 += row[2 * q] * bt[q * 64 + lane];
+            tp1 += row[2 * q + 1] * bb[q * 64 + lane];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < m; ++i) v[i] = v[i] - um1 * W[i] - tp1 * Vs[i];
+    if (k == 0) v[0] = bnd;
+}
+
+template <int N, int DIR, int P>
+__global__ __launch_bounds__(64 * P) void lod_yz_part(float* __restrict__ p, const float* __restrict__ tab,
+                                                      float side, float bnd) {
+    constexpr int64_t NN = (int64_t)N * N;
+    constexpr int m = N / P;
+    __shared__ float bt[P * 64], bb[P * 64];
+    const int lane = threadIdx.x & 63;
+    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t L = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = L < NN;
+    const int64_t Lc = active ? L : 0;
+    const int64_t base = (DIR == 1) ? (Lc % N) + NN * (Lc / N) : Lc;
+    constexpr int stride_b = (int)((DIR == 1) ? N * 4 : NN * 4);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(p, NN * N);
+    const int voff = (int)(base * 4);
+    const int so0 = k * m * stride_b;
+    float v[m];
+#pragma unroll
+    for (int i = 0; i < m; ++i)
+        v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, so0 + i * stride_b, 0));
+    if (k == 0) v[0] = bnd;
+    if (k == P - 1) v[m - 1] = bnd;
+    part_solve<N, P>(v, tab, k, side, bnd, bt, bb, lane);
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < m; ++i)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[i]), rs, voff, so0 + i * stride_b, 0);
+}
+
+// x sweep with fused boundary + uptake, P waves per 64 lines; each wave
+// loads its m-column segment through the LDS transpose tile.
+template <int N, int P>
+__global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, const float* __restrict__ rate,
+                                                     const float* __restrict__ tab, float side, float bnd, float dt) {
+    constexpr int m = N / P;
+    static_assert(m % kChunk == 0, "segment must be a whole number of chunks");
+    constexpr int NC = m / kChunk;
+    constexpr int Q = kChunk / 4;
+    constexpr int LPI = 64 / Q;
+    constexpr int64_t NN = (int64_t)N * N;
+    __shared__ float tiles[P][64 * kTileLd];
+    __shared__ float bt[P * 64], bb[P * 64];
+    const int lane = threadIdx.x & 63;
+    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* tile = tiles[k];
+    const int64_t L0 = (int64_t)blockIdx.x * 64;
+    const __amdgpu_buffer_rsrc_t rp = field_rsrc(p, NN * N);
+    const __amdgpu_buffer_rsrc_t rr = field_rsrc(rate, NN * N);
+    const int lq = lane / Q, part = lane % Q;
+    const int voff = (lq * N + part * 4) * 4;
+    const int sbase = (int)(L0 * N * 4) + k * m * 4;
+    float v[m];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int it = 0; it < Q; ++it) {
+            const int l = it * LPI + lq;
+            const int64_t L = L0 + l;
+            const int j = (int)(L % N), kk = (int)(L / N);
+            const int i0 = k * m + c * kChunk + part * 4;
+            const int so = sbase + (it * LPI * N + c * kChunk) * 4;
+            const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
+            const float4 r = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+            const float av[4] = {a.x, a.y, a.z, a.w};
+            const float rv[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float x = av[e];
+                if (on_boundary(i0 + e, j, kk, N)) x = bnd;
+                x = x / (1.f - rv[e] * dt);
+                tile[l * kTileLd + part * 4 + e] = x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) v[c * kChunk + t] = tile[lane * kTileLd + t];
+        __syncthreads();
+    }
+    if (k == 0) v[0] = bnd;
+    if (k == P - 1) v[m - 1] = bnd;
+    part_solve<N, P>(v, tab, k, side, bnd, bt, bb, lane);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) tile[lane * kTileLd + t] = v[c * kChunk + t];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < Q; ++it) {
+            const int l = it * LPI + lq;
+            const float* src = &tile[l * kTileLd + part * 4];
+            // 16-byte stores go through global (FLAT) stores: the compiler's
+            // hazard model covers a VALU overwrite of their data registers.
+            *reinterpret_cast<float4*>(p + (L0 + l) * N + k * m + c * kChunk + part * 4) =
+                make_float4(src[0], src[1], src[2], src[3]);
+        }
+        __syncthreads();
+    }
 }
 
 // Any N, any direction (dir 0 = x with fused boundary + uptake).
@@ -229,6 +389,54 @@ hipError_t launch_reg(float* p, const float* rate, const float* E, const float* 
     return hipGetLastError();
 }
 
+template <int N, int P>
+void launch_x_part(float* p, const float* rate, const float* tab, float side, float bnd, float dt, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    hipLaunchKernelGGL((lod_x_part<N, P>), dim3(blocks), dim3(64 * P), 0, st, p, rate, tab, side, bnd, dt);
+}
+
+template <int N, int DIR, int P>
+void launch_yz_part(float* p, const float* tab, float side, float bnd, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    hipLaunchKernelGGL((lod_yz_part<N, DIR, P>), dim3(blocks), dim3(64 * P), 0, st, p, tab, side, bnd);
+}
+
+// One substep with per-sweep choices: xp / yzp = 1 (serial line per lane),
+// 2, 4 or 8 (partitioned; a choice the grid does not support -- N / P not a
+// whole number of 32-voxel chunks -- falls back to the next smaller one);
+// spill selects the LDS-spilled serial y/z kernel.
+template <int N>
+hipError_t launch_mixed(float* p, const float* rate, const float* E, const float* F, float* const* tabs,
+                        float side, float bnd, float dt, int xp, int yzp, int spill, hipStream_t st) {
+    constexpr bool ok2 = N % (2 * kChunk) == 0, ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0;
+    const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    if (xp >= 8 && ok8)
+        launch_x_part<N, ok8 ? 8 : 2>(p, rate, tabs[2], side, bnd, dt, st);
+    else if (xp >= 4 && ok4)
+        launch_x_part<N, ok4 ? 4 : 2>(p, rate, tabs[1], side, bnd, dt, st);
+    else if (xp >= 2 && ok2)
+        launch_x_part<N, 2>(p, rate, tabs[0], side, bnd, dt, st);
+    else
+        hipLaunchKernelGGL((lod_x_reg<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
+    if (yzp >= 8 && ok8) {
+        launch_yz_part<N, 1, ok8 ? 8 : 2>(p, tabs[2], side, bnd, st);
+        launch_yz_part<N, 2, ok8 ? 8 : 2>(p, tabs[2], side, bnd, st);
+    } else if (yzp >= 4 && ok4) {
+        launch_yz_part<N, 1, ok4 ? 4 : 2>(p, tabs[1], side, bnd, st);
+        launch_yz_part<N, 2, ok4 ? 4 : 2>(p, tabs[1], side, bnd, st);
+    } else if (yzp >= 2 && ok2) {
+        launch_yz_part<N, 1, 2>(p, tabs[0], side, bnd, st);
+        launch_yz_part<N, 2, 2>(p, tabs[0], side, bnd, st);
+    } else if (spill) {
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    } else {
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
@@ -240,6 +448,14 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     const float side = ctx->side[s], bnd = ctx->P.sub[s].boundary, dt = ctx->P.dt_diff;
     hipStream_t st = ctx->stream;
     const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
+    const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
+    const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
+    if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256)) {
+        float* const* tabs = ctx->ptab[s];
+        if (N == 128) return launch_mixed<128>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        if (N == 192) return launch_mixed<192>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        return launch_mixed<256>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+    }
     switch (N) {
         case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, spill, st);
         case 64: return launch_reg<64>(p, rate, E, F, side, bnd, dt, spill, st);
@@ -254,6 +470,165 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     for (int dir = 0; dir < 3; ++dir)
         hipLaunchKernelGGL(lod_sweep_any, dim3(blocks), dim3(256), 0, st, p, rate, E, F, side, bnd, dt, N, dir);
     return hipGetLastError();
+}
+
+namespace {
+
+// Tables of the partitioned solve (see part_solve) for P blocks, in double.
+std::vector<float> part_table(int N, int P, double side, double corner, double diag) {
+    const int m = N / P;
+    std::vector<double> sub(N), dia(N), sup(N);
+    for (int i = 0; i < N; ++i) {
+        sub[i] = (i == 0 || i == N - 1) ? 0.0 : side;
+        dia[i] = i == 0 ? corner : (i == N - 1 ? 1.0 : diag);
+        sup[i] = i == N - 1 ? 0.0 : side;
+    }
+    std::vector<double> El(N), Fl(N), W(N, 0.0), Vs(N, 0.0);
+    auto local_solve = [&](int k, std::vector<double>& rhs) {  // in place, block k
+        const int a = k * m;
+        std::vector<double> Y(m);
+        Y[0] = rhs[0];
+        for (int i = 1; i < m; ++i) Y[i] = rhs[i] - El[a + i] * Y[i - 1];
+        rhs[m - 1] = Y[m - 1] / Fl[a + m - 1];
+        for (int i = m - 2; i >= 0; --i) rhs[i] = (Y[i] - sup[a + i] * rhs[i + 1]) / Fl[a + i];
+    };
+    for (int k = 0; k < P; ++k) {
+        const int a = k * m;
+        El[a] = 0.0;
+        Fl[a] = dia[a];
+        for (int i = 1; i < m; ++i) {
+            El[a + i] = sub[a + i] / Fl[a + i - 1];
+            Fl[a + i] = dia[a + i] - El[a + i] * sup[a + i - 1];
+        }
+        std::vector<double> w(m, 0.0), v(m, 0.0);
+        if (k > 0) {
+            w[0] = sub[a];
+            local_solve(k, w);
+        }
+        if (k < P - 1) {
+            v[m - 1] = sup[a + m - 1];
+            local_solve(k, v);
+        }
+        for (int i = 0; i < m; ++i) {
+            W[a + i] = w[i];
+            Vs[a + i] = v[i];
+        }
+    }
+    // reduced system R z = g, z = (t0, u0, t1, u1, ...)
+    const int R2 = 2 * P;
+    std::vector<double> R(R2 * R2, 0.0), Ri(R2 * R2, 0.0);
+    for (int k = 0; k < P; ++k) {
+        const int a = k * m;
+        R[(2 * k) * R2 + 2 * k] = 1.0;
+        R[(2 * k + 1) * R2 + 2 * k + 1] += 1.0;
+        if (k > 0) {
+            R[(2 * k) * R2 + 2 * (k - 1) + 1] += W[a];
+            R[(2 * k + 1) * R2 + 2 * (k - 1) + 1] += W[a + m - 1];
+        }
+        if (k < P - 1) {
+            R[(2 * k) * R2 + 2 * (k + 1)] += Vs[a];
+            R[(2 * k + 1) * R2 + 2 * (k + 1)] += Vs[a + m - 1];
+        }
+    }
+    for (int i = 0; i < R2; ++i) Ri[i * R2 + i] = 1.0;
+    for (int c = 0; c < R2; ++c) {  // Gauss-Jordan with partial pivoting
+        int piv = c;
+        for (int r = c + 1; r < R2; ++r)
+            if (std::fabs(R[r * R2 + c]) > std::fabs(R[piv * R2 + c])) piv = r;
+        for (int q = 0; q < R2; ++q) {
+            std::swap(R[c * R2 + q], R[piv * R2 + q]);
+            std::swap(Ri[c * R2 + q], Ri[piv * R2 + q]);
+        }
+        const double d = R[c * R2 + c];
+        for (int q = 0; q < R2; ++q) {
+            R[c * R2 + q] /= d;
+            Ri[c * R2 + q] /= d;
+        }
+        for (int r = 0; r < R2; ++r) {
+            if (r == c) continue;
+            const double f = R[r * R2 + c];
+            if (f == 0.0) continue;
+            for (int q = 0; q < R2; ++q) {
+                R[r * R2 + q] -= f * R[c * R2 + q];
+                Ri[r * R2 + q] -= f * Ri[c * R2 + q];
+            }
+        }
+    }
+    std::vector<float> t(4 * N + R2 * R2);
+    for (int i = 0; i < N; ++i) {
+        t[i] = (float)El[i];
+        t[N + i] = (float)Fl[i];
+        t[2 * N + i] = (float)W[i];
+        t[3 * N + i] = (float)Vs[i];
+    }
+    for (int i = 0; i < R2 * R2; ++i) t[4 * N + i] = (float)Ri[i];
+    return t;
+}
+
+}  // namespace
+
+// Times every effective LOD variant on a scratch copy of substrate 0's field
+// (the context's own field and uptake map are left as they were) and sets
+// the fastest. bit_exact restricts the search to the serial-line kernels.
+hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
+    const size_t bytes = sizeof(float) * (size_t)ctx->V;
+    float* scratch = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&scratch), bytes);
+    if (e != hipSuccess) return e;
+    float* const real = ctx->field[0];
+    e = hipMemcpyAsync(scratch, real, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    const int saved[3] = {ctx->tune[GELL_KNOB_LOD_X_PARTS], ctx->tune[GELL_KNOB_LOD_PARTS],
+                          ctx->tune[GELL_KNOB_LOD_YZ_LDS]};
+    int best[3] = {1, 1, saved[2]};
+    double best_t = 1e30;
+    const bool part_ok = (ctx->N == 128 || ctx->N == 192 || ctx->N == 256) && !bit_exact;
+    const int cand[4] = {1, 2, 4, 8};
+    ctx->field[0] = scratch;
+    for (int xi = 0; xi < (part_ok ? 4 : 1) && e == hipSuccess; ++xi)
+        for (int yi = 0; yi < (part_ok ? 4 : 1) && e == hipSuccess; ++yi)
+            for (int sp = 0; sp < 2 && e == hipSuccess; ++sp) {
+                if (cand[yi] > 1 && sp == 1) continue;  // the spill only exists for serial y/z
+                const int xp = cand[xi], yzp = cand[yi];
+                if ((xp > 1 && ctx->N % (xp * 32) != 0) || (yzp > 1 && ctx->N % (yzp * 32) != 0)) continue;
+                ctx->tune[GELL_KNOB_LOD_X_PARTS] = xp;
+                ctx->tune[GELL_KNOB_LOD_PARTS] = yzp;
+                ctx->tune[GELL_KNOB_LOD_YZ_LDS] = sp;
+                for (int w = 0; w < 2 && e == hipSuccess; ++w) e = lod_substep_launch(ctx, 0);
+                if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+                const int reps = 5;
+                for (int r = 0; r < reps && e == hipSuccess; ++r) e = lod_substep_launch(ctx, 0);
+                if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                const double us = 1e3 * ms / reps;
+                if (e == hipSuccess && us < best_t) {
+                    best_t = us;
+                    best[0] = xp;
+                    best[1] = yzp;
+                    best[2] = sp;
+                }
+            }
+    ctx->field[0] = real;
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = e2;
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(scratch);
+    if (e == hipSuccess) {
+        ctx->tune[GELL_KNOB_LOD_X_PARTS] = best[0];
+        ctx->tune[GELL_KNOB_LOD_PARTS] = best[1];
+        ctx->tune[GELL_KNOB_LOD_YZ_LDS] = best[2];
+        if (best_us) *best_us = best_t;
+    } else {
+        ctx->tune[GELL_KNOB_LOD_X_PARTS] = saved[0];
+        ctx->tune[GELL_KNOB_LOD_PARTS] = saved[1];
+        ctx->tune[GELL_KNOB_LOD_YZ_LDS] = saved[2];
+    }
+    return e;
 }
 
 // Mesh_struct::Mesh_initialize, LODkernel.cuh:239-259: the Thomas
@@ -278,6 +653,20 @@ hipError_t lod_init_coeffs(gell_ctx* ctx, int s) {
     F[N - 1] = corner - prod;
     volatile float ts = E[1] * F[0];  // thomas_side, LODkernel.cuh:39
     ctx->side[s] = ts;
+    const double side64 = -(double)dt * sb.D / ((double)dx * dx);
+    const double corner64 = 1.0 + (double)dt * sb.decay / 3.0 - side64;
+    const double diag64 = corner64 - side64;
+    // The host tables must outlive the asynchronous copies: they are kept
+    // until the stream synchronisation below.
+    const int parts[3] = {2, 4, 8};
+    std::vector<float> tabs[3];
+    for (int q = 0; q < 3; ++q) {
+        if (N % (parts[q] * kChunk) != 0) continue;
+        tabs[q] = part_table(N, parts[q], side64, corner64, diag64);
+        hipError_t e = hipMemcpyAsync(ctx->ptab[s][q], tabs[q].data(), tabs[q].size() * sizeof(float),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return e;
+    }
     hipError_t e = hipMemcpyAsync(ctx->E[s], E.data(), N * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(ctx->F[s], F.data(), N * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
@@ -286,3 +675,17 @@ hipError_t lod_init_coeffs(gell_ctx* ctx, int s) {
 }
 
 }  // namespace gell
+
+extern "C" int gell_lod_partition_table(const gell_params* P, int32_t substrate, int32_t parts, float* out,
+                                        int32_t out_len) {
+    if (!P || !out || substrate < 0 || substrate >= P->n_substrates) return GELL_E_ARG;
+    if (parts < 2 || P->N % parts != 0 || P->N / parts < 2) return GELL_E_ARG;
+    const int N = P->N;
+    if (out_len < 4 * N + 4 * parts * parts) return GELL_E_ARG;
+    const gell_substrate& sb = P->sub[substrate];
+    const double side = -(double)P->dt_diff * sb.D / ((double)P->dx * P->dx);
+    const double corner = 1.0 + (double)P->dt_diff * sb.decay / 3.0 - side;
+    const std::vector<float> t = gell::part_table(N, parts, side, corner, corner - side);
+    for (size_t i = 0; i < t.size(); ++i) out[i] = t[i];
+    return GELL_OK;
+}

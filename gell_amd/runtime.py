@@ -111,6 +111,8 @@ def load_library() -> C.CDLL:
         "gell_profile_enable": (C.c_int, [vp, C.c_int]),
         "gell_tune": (C.c_int, [vp, C.c_int, C.c_int]),
         "gell_tune_get": (C.c_int, [vp, C.c_int]),
+        "gell_autotune": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "gell_lod_partition_table": (C.c_int, [P(Params), i32, i32, P(C.c_float), i32]),
         "gell_profile_read": (C.c_int, [vp, P(C.c_double), P(i64), C.c_int]),
         "gell_dev_alloc": (vp, [sz]),
         "gell_dev_free": (None, [vp]),
@@ -318,13 +320,22 @@ class Context:
         return download_ptr(self.lib.gell_birth_slots(self.h), np.int32, n)
 
     # -- kernel variants -------------------------------------------------------
-    KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1}
+    KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1, "lod_parts": 2, "lod_x_parts": 3}
 
     def tune(self, knob: str, value: int) -> None:
         self._check(self.lib.gell_tune(self.h, self.KNOBS[knob], int(value)), "gell_tune")
 
     def tune_get(self, knob: str) -> int:
         return self.lib.gell_tune_get(self.h, self.KNOBS[knob])
+
+    def autotune(self, bit_exact: bool = False) -> dict:
+        """Pick the fastest kernel variants for this context's grid; returns
+        the chosen knob values and the best LOD substep time."""
+        us = C.c_double()
+        self._check(self.lib.gell_autotune(self.h, 1 if bit_exact else 0, C.byref(us)), "gell_autotune")
+        out = {k: self.tune_get(k) for k in self.KNOBS}
+        out["lod_us"] = us.value
+        return out
 
     # -- profiling -------------------------------------------------------------
     def profile(self, enable: bool = True) -> None:
@@ -350,6 +361,17 @@ def float_loop_count(end: float, step: float) -> int:
         c += 1
         t = np.float32(t + step)
     return c
+
+
+def lod_partition_table(params: Params, substrate: int, parts: int) -> np.ndarray:
+    """Host-built coefficient table of the partitioned line solve."""
+    n = 4 * params.N + 4 * parts * parts
+    out = np.zeros(n, np.float32)
+    st = load_library().gell_lod_partition_table(C.byref(params), substrate, parts,
+                                                 out.ctypes.data_as(C.POINTER(C.c_float)), n)
+    if st != 0:
+        raise GellError(f"gell_lod_partition_table: {st}")
+    return out
 
 
 def device_name() -> str:

@@ -156,15 +156,43 @@ const int32_t* gell_birth_slots(gell_ctx* ctx);
 
 int gell_sync(gell_ctx* ctx);
 
-/* Kernel-variant knobs (the autotuner's search space). Every value of every
- * knob gives bit-identical results; only speed differs.
+/* Kernel-variant knobs (the autotuner's search space).
  *   GELL_KNOB_LOD_YZ_LDS   0 = y/z sweep lines fully in registers,
  *                          1 = first 64 forward values per line in LDS
  *                              (two waves per SIMD), for N > 64
  *   GELL_KNOB_FM_PREFETCH  0 = one stencil-range load per voxel in turn,
- *                          1 = all 27 stencil ranges loaded up front */
-enum { GELL_KNOB_LOD_YZ_LDS = 0, GELL_KNOB_FM_PREFETCH = 1, GELL_N_KNOBS = 2 };
+ *                          1 = all 27 stencil ranges loaded up front
+ *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
+ *                              reference's serial recurrence, bit-identical
+ *                              to it), 2, 4 or 8 = each line split into
+ *                              blocks solved by separate waves and joined
+ *                              through the exact reduced system (same linear
+ *                              solve, rounding differs at the 1e-7 relative
+ *                              level); applies for N in {128, 192, 256}, a
+ *                              block count the grid cannot split into
+ *                              32-voxel chunks falls back to a smaller one.
+ *                              Default 4.
+ *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep. Default 4.
+ * The first two knobs never change results; gell_autotune sets all four. */
+enum {
+    GELL_KNOB_LOD_YZ_LDS = 0,
+    GELL_KNOB_FM_PREFETCH = 1,
+    GELL_KNOB_LOD_PARTS = 2,
+    GELL_KNOB_LOD_X_PARTS = 3,
+    GELL_N_KNOBS = 4
+};
 int gell_tune(gell_ctx* ctx, int knob, int value);
+/* Time every effective LOD variant on a scratch copy of substrate 0's field
+ * (state is not changed) and keep the fastest; with GELL_TUNE_BIT_EXACT only
+ * the variants bit-identical to the serial sweep are considered. The best
+ * substep time (microseconds) is written to *lod_us_out when not NULL.
+ * Synchronises; allocates one field-sized scratch buffer while it runs. */
+#define GELL_TUNE_BIT_EXACT 1
+int gell_autotune(gell_ctx* ctx, int flags, double* lod_us_out);
+/* Coefficient table of the partitioned line solve for `parts` blocks
+ * (GELL_KNOB_LOD_PARTS), host-side: El[N] | Fl[N] | W[N] | V[N] |
+ * Rinv[(2 parts)^2]. For inspection and CPU-side checks; needs no device. */
+int gell_lod_partition_table(const gell_params* params, int32_t substrate, int32_t parts, float* out, int32_t out_len);
 int gell_tune_get(const gell_ctx* ctx, int knob);
 
 /* Per-phase device time, recorded with HIP events on the context stream

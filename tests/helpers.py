@@ -90,11 +90,15 @@ def compare_cells(got, want, n, float_tol=1e-5, exact=True, skip=()):
 
 
 class GpuState:
-    """Context + device cell array for one parameter set."""
+    """Context + device cell array for one parameter set. With exact=True the
+    LOD uses the serial line kernels, which are bit-identical to the oracle."""
 
-    def __init__(self, P, cells: np.ndarray):
+    def __init__(self, P, cells: np.ndarray, exact: bool = False):
         self.P = P
         self.ctx = rt.Context(P)
+        if exact:
+            self.ctx.tune("lod_parts", 1)
+            self.ctx.tune("lod_x_parts", 1)
         self.buf = rt.DeviceBuffer(P.capacity * 72)
         self.upload(cells)
 
