@@ -64,8 +64,15 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ driver with the reference's time-loop structure, linked to the library
+    sim = OUT / "gell_sim"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-o", str(sim), str(CSRC / "gell_sim.cpp"),
+           f"-L{OUT}", "-lgell_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gell_sim build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {sim}")
     return LIB
 
 

@@ -99,6 +99,7 @@ void release(gell_ctx* c) {
     dfree(c->sorted);
     dfree(c->nb);
     dfree(c->se);
+    dfree(c->se_lin);
     dfree(c->se_list);
     for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) {
         dfree(c->field[s]);
@@ -182,6 +183,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     (void)hipGetDevice(&c->dev);
     c->N = P.N;
     c->V = (int64_t)P.N * P.N * P.N;
+    c->KS = (int64_t)1 << (3 * gell::ceil_log2(P.N));
     const size_t cap = (size_t)P.capacity;
     const int64_t tiles = ((int64_t)cap + 4095) / 4096;
     c->hist_cap = 512 * tiles;
@@ -201,7 +203,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->out_order, cap));
     A(dalloc(&c->sorted, cap));
     A(dalloc(&c->nb, cap));
-    A(dalloc(&c->se, (size_t)c->V));
+    A(dalloc(&c->se, (size_t)c->KS));
     A(dalloc(&c->se_list, cap));
     for (int s = 0; s < P.n_substrates; ++s) {
         A(dalloc(&c->field[s], (size_t)c->V));
@@ -336,7 +338,17 @@ float* gell_uptake_field(gell_ctx* ctx, int32_t s) {
 }
 const int32_t* gell_sorted_keys(gell_ctx* ctx) { return ctx ? ctx->out_keys : nullptr; }
 const int32_t* gell_sorted_order(gell_ctx* ctx) { return ctx ? ctx->out_order : nullptr; }
-const int32_t* gell_voxel_ranges(gell_ctx* ctx) { return ctx ? reinterpret_cast<const int32_t*>(ctx->se) : nullptr; }
+const int32_t* gell_voxel_ranges(gell_ctx* ctx) {
+    if (!ctx) return nullptr;
+    if (!ctx->se_lin && dalloc(&ctx->se_lin, (size_t)ctx->V) != hipSuccess) {
+        ctx->se_lin = nullptr;
+        fail(ctx, GELL_E_NOMEM, "voxel range buffer");
+        return nullptr;
+    }
+    if (gell::fm_ranges_linear(ctx, ctx->se_lin) != hipSuccess) return nullptr;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
+    return reinterpret_cast<const int32_t*>(ctx->se_lin);
+}
 const int32_t* gell_birth_slots(gell_ctx* ctx) { return ctx ? ctx->slots : nullptr; }
 
 int gell_sync(gell_ctx* ctx) {
@@ -349,7 +361,8 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
-    const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8) : (value == 0 || value == 1);
+    const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8)
+                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && (value == 2 || value == 3)));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     ctx->tune[knob] = value;
     return GELL_OK;

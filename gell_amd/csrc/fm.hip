@@ -14,6 +14,11 @@
 //                key kernel's record updates (sign, mesh_idx), builds the
 //                per-voxel live ranges with atomic min/max (order-independent
 //                result) and the compact (x, y, z, r) neighbour array.
+//                The ranges are indexed by Morton code, not by the linear
+//                voxel index: cells arrive in Morton order, so both the
+//                atomics and the force kernel's 27-voxel stencil lookups fall
+//                into a few compact bricks of the array. gell_voxel_ranges
+//                converts them to the reference's linear layout on request.
 //   fm_force_move  forcekernel + movementkernel fused: reads the sorted
 //                record once, visits the 27-voxel stencil in the reference's
 //                order (x outer, z inner, ascending slot) through the compact
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
         c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
     store_cell(sorted, i, c);
     const bool live = csign(c);
-    const int32_t v = live ? (int32_t)c.w[W_MESH] : -1;
+    const int32_t v = live ? (int32_t)morton3(a, b, k) : -1;
     se_list[i] = v;
     nb[i] = make_float4(px, py, pz, live ? cf(c, W_R) : __uint_as_float(kDeadTag));
     if (live) {
@@ -304,6 +309,65 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
 // ---------------------------------------------------------------------------
 // Force + movement
 // ---------------------------------------------------------------------------
+// One candidate pair of forcekernel (FMkernel.cuh:86-109): returns false
+// when the pair is skipped; thres carries over between calls.
+__device__ __forceinline__ void force_pair(float4 o, float p1x, float p1y, float p1z, float r1, float& thres,
+                                           float& fx, float& fy, float& fz) {
+    if (__float_as_uint(o.w) == kDeadTag) return;
+    const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
+    const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+    const float distance = sqrtf((float)((double)ss + 0.0001));
+    if (distance >= thres) return;
+    const float rsum = o.w + r1;
+    const float R = rsum > 0.01f ? rsum : 0.01f;
+    thres = (float)((double)R * 1.25);
+    if (distance >= thres) return;
+    const float qr = 1.f - distance / R;
+    const float qa = 1.f - distance / thres;
+    const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
+    const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
+    const float cc = coefr - coefa;
+    fx += ddx * cc;
+    fy += ddy * cc;
+    fz += ddz * cc;
+}
+
+// Grouped variant: all 27 ranges are loaded at once; then, for each of the
+// three x-planes of the stencil, the first two candidates of its nine voxels
+// are loaded together before the plane's pairs are evaluated in order. A
+// voxel's third and later candidates are loaded as they are reached. Each
+// wave thus waits on ~5 memory round trips instead of one per candidate.
+__device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb, const int2* __restrict__ se,
+                                                   int32_t N, float dx, float p1x, float p1y, float p1z, float r1,
+                                                   float& fx, float& fy, float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    int2 rr[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+        rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
+    }
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        float4 c0[9], c1[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int2 r = rr[g * 9 + t];
+            c0[t] = r.y >= 0 ? nb[r.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c1[t] = r.y > r.x ? nb[r.x + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int2 r = rr[g * 9 + t];
+            if (r.y < 0) continue;
+            force_pair(c0[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            if (r.y > r.x) force_pair(c1[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            for (int j = r.x + 2; j <= r.y; ++j) force_pair(nb[j], p1x, p1y, p1z, r1, thres, fx, fy, fz);
+        }
+    }
+}
+
 // Force on live cell i (forcekernel, FMkernel.cuh:56-118). With PREFETCH the
 // 27 stencil ranges are loaded before the pair loop, so their latencies
 // overlap; the visiting order (x outer, z inner, ascending slot) is the same.
@@ -318,7 +382,7 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
-            rr[q] = inside(xi, yi, zi, N) ? se[lin(xi, yi, zi, N)] : make_int2(0, -1);
+            rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
         }
     }
 #pragma unroll
@@ -329,7 +393,7 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
             r = rr[q];
         } else {
             if (!inside(xi, yi, zi, N)) continue;
-            r = se[lin(xi, yi, zi, N)];
+            r = se[morton3(xi, yi, zi)];
         }
         if (r.y < 0) continue;
         for (int j = r.x; j <= r.y; ++j) {
@@ -356,18 +420,92 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
     }
 }
 
-template <bool PREFETCH>
+// Flattened variant: each lane keeps a 27-bit mask of its non-empty stencil
+// voxels and walks its own candidate list (voxel by voxel in the reference's
+// order, slots ascending); every iteration of the wave's loop evaluates one
+// pair per active lane. The loop runs for the largest candidate count in the
+// wave instead of for every (voxel, candidate) position any lane occupies.
+// All lanes of the wave must call it (the loop exit is a wave vote); lanes
+// with live == false have no candidates and load nothing.
+__device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, const int2* __restrict__ se,
+                                                int32_t N, float dx, bool live, float p1x, float p1y, float p1z,
+                                                float r1, float& fx, float& fy, float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    uint32_t mask = 0;
+    if (live) {
+        int2 rr[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+            rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
+        }
+#pragma unroll
+        for (int q = 0; q < 27; ++q) mask |= (rr[q].y >= 0 ? 1u : 0u) << q;
+    }
+    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    int j = 0, je = -1;
+    while (true) {
+        if (j > je && mask != 0) {
+            const int q = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+            const int2 r = se[morton3(xi, yi, zi)];
+            j = r.x;
+            je = r.y;
+        }
+        const bool active = j <= je;
+        if (!__any(active)) break;
+        if (!active) continue;
+        const float4 o = nb[j];
+        ++j;
+        if (__float_as_uint(o.w) == kDeadTag) continue;
+        // FMkernel.cuh:89-109
+        const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
+        const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+        const float distance = sqrtf((float)((double)ss + 0.0001));
+        if (distance >= thres) continue;
+        const float rsum = o.w + r1;
+        const float R = rsum > 0.01f ? rsum : 0.01f;
+        thres = (float)((double)R * 1.25);
+        if (distance >= thres) continue;
+        const float qr = 1.f - distance / R;
+        const float qa = 1.f - distance / thres;
+        const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
+        const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
+        const float cc = coefr - coefa;
+        fx += ddx * cc;
+        fy += ddy * cc;
+        fz += ddz * cc;
+    }
+}
+
+// VARIANT 0: stencil voxels in turn; 1: their ranges loaded up front;
+// 2: flattened per-lane candidate walk; 3: grouped candidate loads. All four
+// give the same bits.
+template <int VARIANT>
 __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
                                                         const float4* __restrict__ nb, const int2* __restrict__ se,
                                                         gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
                                                         float dt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    CellRegs c = load_cell(sorted, i);
-    if (csign(c)) {
+    const bool in_range = i < n;
+    CellRegs c;
+    if (in_range) c = load_cell(sorted, i);
+    const bool live = in_range && csign(c);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (VARIANT == 2) {
+        // every lane of the wave takes part in the candidate loop's vote
+        const float p1x = live ? cf(c, W_PX) : 0.f, p1y = live ? cf(c, W_PY) : 0.f,
+                    p1z = live ? cf(c, W_PZ) : 0.f;
+        cell_force_flat(nb, se, N, dx, live, p1x, p1y, p1z, live ? cf(c, W_R) : 0.f, fx, fy, fz);
+    } else if (VARIANT == 3) {
+        if (live) cell_force_grouped(nb, se, N, dx, cf(c, W_PX), cf(c, W_PY), cf(c, W_PZ), cf(c, W_R), fx, fy, fz);
+    } else if (live) {
+        cell_force<VARIANT == 1>(nb, se, N, dx, cf(c, W_PX), cf(c, W_PY), cf(c, W_PZ), cf(c, W_R), fx, fy, fz);
+    }
+    if (!in_range) return;
+    if (live) {
         const float p1x = cf(c, W_PX), p1y = cf(c, W_PY), p1z = cf(c, W_PZ);
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        cell_force<PREFETCH>(nb, se, N, dx, p1x, p1y, p1z, cf(c, W_R), fx, fy, fz);
         // movementkernel, FMkernel.cuh:130-158
         const float h = dt / 2.f;
         const float ox = cf(c, W_OX), oy = cf(c, W_OY), oz = cf(c, W_OZ);
@@ -401,6 +539,15 @@ __global__ void se_fill(int2* se, int64_t V, int32_t s_empty) {
     if (v < V) se[v] = make_int2(s_empty, -1);
 }
 
+// Morton-indexed ranges -> linear (x fastest) layout of the reference.
+__global__ void se_to_linear(const int2* __restrict__ se, int2* __restrict__ out, int32_t N) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t V = (int64_t)N * N * N;
+    if (v >= V) return;
+    const int32_t i = (int32_t)(v % N), j = (int32_t)((v / N) % N), k = (int32_t)(v / ((int64_t)N * N));
+    out[v] = se[morton3(i, j, k)];
+}
+
 }  // namespace
 
 SortPlan plan_sort(int key_bits) {
@@ -416,9 +563,15 @@ SortPlan plan_sort(int key_bits) {
 }
 
 hipError_t fm_init_ranges(gell_ctx* ctx) {
-    const unsigned blocks = (unsigned)((ctx->V + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(se_fill, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, ctx->V, ctx->P.capacity + 1);
+    const unsigned blocks = (unsigned)((ctx->KS + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(se_fill, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, ctx->KS, ctx->P.capacity + 1);
     ctx->se_list_n = 0;
+    return hipGetLastError();
+}
+
+hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out) {
+    const unsigned blocks = (unsigned)((ctx->V + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(se_to_linear, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, out, ctx->N);
     return hipGetLastError();
 }
 
@@ -458,11 +611,18 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kBlock - 1) / kBlock);
-    if (ctx->tune[GELL_KNOB_FM_PREFETCH])
-        hipLaunchKernelGGL(fm_force_move<true>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+    const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    if (v == 3)
+        hipLaunchKernelGGL(fm_force_move<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    else if (v == 2)
+        hipLaunchKernelGGL(fm_force_move<2>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    else if (v == 1)
+        hipLaunchKernelGGL(fm_force_move<1>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
                            cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
     else
-        hipLaunchKernelGGL(fm_force_move<false>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+        hipLaunchKernelGGL(fm_force_move<0>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
                            cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
     return hipGetLastError();
 }

@@ -27,7 +27,9 @@ struct gell_ctx {
     int32_t* out_order = nullptr;  // permutation of the last fm_update
     gell_cell* sorted = nullptr;   // records in key order
     float4* nb = nullptr;          // (x, y, z, r or dead tag) per sorted slot
-    int2* se = nullptr;            // per voxel (s, e)
+    int2* se = nullptr;            // per voxel (s, e), indexed by Morton code
+    int2* se_lin = nullptr;        // linear-layout copy for gell_voxel_ranges
+    int64_t KS = 0;                // Morton key space, 2^(3 ceil(log2 N))
     int32_t* se_list = nullptr;    // voxels written by the last fm_update
     int32_t se_list_n = 0;
 
@@ -52,7 +54,7 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 1, 4, 4};
+    int tune[GELL_N_KNOBS] = {1, 3, 4, 4};
 
     // Profiling.
     bool prof = false;
@@ -80,6 +82,7 @@ hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n);
 hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n);
 hipError_t fm_init_ranges(gell_ctx* ctx);
+hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out);
 
 hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n);
 

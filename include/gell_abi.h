@@ -144,8 +144,11 @@ float* gell_uptake_field(gell_ctx* ctx, int32_t substrate); /* N^3 floats (p_rat
 /* Cell-list state of the last gell_fm_update, device pointers:
  * sorted keys (Morton code, -1 outside; n entries), the permutation (sorted
  * slot -> input slot; n entries), and the per-voxel live ranges as
- * interleaved pairs (s[v], e[v]) for v < N^3 -- (capacity+1, -1) when the
- * voxel holds no live cell (Mech_Mesh_s / Mech_Mesh_e, FMkernel.cuh:38-53). */
+ * interleaved pairs (s[v], e[v]) for v < N^3 in the linear (x fastest)
+ * layout -- (capacity+1, -1) when the voxel holds no live cell
+ * (Mech_Mesh_s / Mech_Mesh_e, FMkernel.cuh:38-53). The library keeps the
+ * ranges indexed by Morton code; gell_voxel_ranges converts them into a
+ * context-owned buffer (allocated on first use), synchronises, returns it. */
 const int32_t* gell_sorted_keys(gell_ctx* ctx);
 const int32_t* gell_sorted_order(gell_ctx* ctx);
 const int32_t* gell_voxel_ranges(gell_ctx* ctx);
@@ -161,7 +164,13 @@ int gell_sync(gell_ctx* ctx);
  *                          1 = first 64 forward values per line in LDS
  *                              (two waves per SIMD), for N > 64
  *   GELL_KNOB_FM_PREFETCH  0 = one stencil-range load per voxel in turn,
- *                          1 = all 27 stencil ranges loaded up front
+ *                          1 = all 27 stencil ranges loaded up front,
+ *                          2 = each lane walks its own candidate list (a
+ *                              27-bit mask of non-empty voxels), one pair per
+ *                              lane per step of the wave's loop,
+ *                          3 = per x-plane of the stencil, the first two
+ *                              candidates of its nine voxels loaded together.
+ *                              Default 3.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4 or 8 = each line split into

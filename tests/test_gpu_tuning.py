@@ -33,7 +33,7 @@ def test_fm_variants_identical():
     P = params(N, capacity=n)
     cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
     outs = []
-    for v in (0, 1):
+    for v in (0, 1, 2, 3):
         st = GpuState(P, cells)
         st.ctx.tune("fm_prefetch", v)
         for _ in range(3):
@@ -41,6 +41,8 @@ def test_fm_variants_identical():
         outs.append(st.cells())
         st.close()
     assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
+    assert np.array_equal(outs[0].view(np.uint8), outs[2].view(np.uint8))
+    assert np.array_equal(outs[0].view(np.uint8), outs[3].view(np.uint8))
 
 
 def test_bad_knob_is_an_error():
