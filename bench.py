@@ -46,6 +46,9 @@ SUBSTRATES = [
 CONFIGS = {
     "L": dict(N=256, dx=10.0, cells=1_000_000, radius=1200.0, n_sub=2, capacity=2_000_000),
     "S": dict(N=64, dx=10.0, cells=10_000, radius=250.0, n_sub=1, capacity=20_000),
+    # heterogeneous spheroid: ~1 cell per voxel (26 neighbours in the stencil),
+    # proliferating rim, quiescent shell, necrotic core
+    "H": dict(N=512, dx=10.0, cells=10_000_000, radius=1336.0, n_sub=1, capacity=12_000_000, profile=True),
 }
 
 
@@ -60,6 +63,9 @@ def make_workload(cfg: dict, seed: int):
     P = rt.make_params(N=cfg["N"], dx=cfg["dx"], capacity=cfg["capacity"], seed=seed,
                        substrates=SUBSTRATES[: cfg["n_sub"]])
     cells = spheroid.make_cells(cfg["cells"], cfg["N"], cfg["dx"], cfg["radius"], seed, capacity=cfg["capacity"])
+    if cfg.get("profile"):
+        c = cfg["N"] * cfg["dx"] / 2.0
+        spheroid.radial_profile(cells, cfg["cells"], (c, c, c), cfg["radius"], seed + 1)
     return P, cells
 
 

@@ -109,6 +109,7 @@ void release(gell_ctx* c) {
         dfree(c->ptab[s][0]);
         dfree(c->ptab[s][1]);
         dfree(c->ptab[s][2]);
+        dfree(c->ptab[s][3]);
         dfree(c->acc[s]);
         dfree(c->rate_list[s]);
     }
@@ -213,6 +214,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         A(dalloc(&c->ptab[s][0], (size_t)(4 * P.N + 16)));
         A(dalloc(&c->ptab[s][1], (size_t)(4 * P.N + 64)));
         A(dalloc(&c->ptab[s][2], (size_t)(4 * P.N + 256)));
+        A(dalloc(&c->ptab[s][3], (size_t)(4 * P.N + 1024)));
         A(dalloc(&c->acc[s], (size_t)c->V));
         A(dalloc(&c->rate_list[s], cap));
     }
@@ -361,7 +363,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
-    const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8)
+    const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
                           : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && (value == 2 || value == 3)));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     ctx->tune[knob] = value;

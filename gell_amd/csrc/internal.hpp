@@ -39,7 +39,7 @@ struct gell_ctx {
     float* E[GELL_MAX_SUBSTRATES] = {};
     float* F[GELL_MAX_SUBSTRATES] = {};
     float side[GELL_MAX_SUBSTRATES] = {};
-    float* ptab[GELL_MAX_SUBSTRATES][3] = {};  // partitioned-solve tables, P = 2, 4, 8
+    float* ptab[GELL_MAX_SUBSTRATES][4] = {};  // partitioned-solve tables, P = 2, 4, 8, 16
     long long* acc[GELL_MAX_SUBSTRATES] = {};
     int32_t* rate_list[GELL_MAX_SUBSTRATES] = {};
     int32_t rate_list_n[GELL_MAX_SUBSTRATES] = {};

@@ -402,23 +402,33 @@ void launch_yz_part(float* p, const float* tab, float side, float bnd, hipStream
 }
 
 // One substep with per-sweep choices: xp / yzp = 1 (serial line per lane),
-// 2, 4 or 8 (partitioned; a choice the grid does not support -- N / P not a
-// whole number of 32-voxel chunks -- falls back to the next smaller one);
-// spill selects the LDS-spilled serial y/z kernel.
+// 2, 4, 8 or 16 (partitioned; a choice the grid does not support -- N / P
+// not a whole number of 32-voxel chunks -- falls back to the next smaller
+// one; the x sweep stops at 8 blocks, whose transpose tiles already take
+// 66 KB of LDS); spill selects the LDS-spilled serial y/z kernel. Above
+// N = 256 a whole line does not fit in registers and the serial choice runs
+// the generic kernel.
 template <int N>
 hipError_t launch_mixed(float* p, const float* rate, const float* E, const float* F, float* const* tabs,
                         float side, float bnd, float dt, int xp, int yzp, int spill, hipStream_t st) {
-    constexpr bool ok2 = N % (2 * kChunk) == 0, ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0;
+    constexpr bool ok2 = N % (2 * kChunk) == 0, ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0,
+                   ok16 = N % (16 * kChunk) == 0;
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    const unsigned gblocks = (unsigned)(((int64_t)N * N + 255) / 256);
     if (xp >= 8 && ok8)
         launch_x_part<N, ok8 ? 8 : 2>(p, rate, tabs[2], side, bnd, dt, st);
     else if (xp >= 4 && ok4)
         launch_x_part<N, ok4 ? 4 : 2>(p, rate, tabs[1], side, bnd, dt, st);
     else if (xp >= 2 && ok2)
         launch_x_part<N, 2>(p, rate, tabs[0], side, bnd, dt, st);
-    else
+    else if constexpr (N <= 256)
         hipLaunchKernelGGL((lod_x_reg<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
-    if (yzp >= 8 && ok8) {
+    else
+        hipLaunchKernelGGL(lod_sweep_any, dim3(gblocks), dim3(256), 0, st, p, rate, E, F, side, bnd, dt, N, 0);
+    if (yzp >= 16 && ok16) {
+        launch_yz_part<N, 1, ok16 ? 16 : 2>(p, tabs[3], side, bnd, st);
+        launch_yz_part<N, 2, ok16 ? 16 : 2>(p, tabs[3], side, bnd, st);
+    } else if (yzp >= 8 && ok8) {
         launch_yz_part<N, 1, ok8 ? 8 : 2>(p, tabs[2], side, bnd, st);
         launch_yz_part<N, 2, ok8 ? 8 : 2>(p, tabs[2], side, bnd, st);
     } else if (yzp >= 4 && ok4) {
@@ -427,12 +437,17 @@ hipError_t launch_mixed(float* p, const float* rate, const float* E, const float
     } else if (yzp >= 2 && ok2) {
         launch_yz_part<N, 1, 2>(p, tabs[0], side, bnd, st);
         launch_yz_part<N, 2, 2>(p, tabs[0], side, bnd, st);
-    } else if (spill) {
-        hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
-        hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    } else if constexpr (N <= 256) {
+        if (spill) {
+            hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+            hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        } else {
+            hipLaunchKernelGGL((lod_yz_reg<N, 1, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+            hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        }
     } else {
-        hipLaunchKernelGGL((lod_yz_reg<N, 1, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
-        hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        for (int dir = 1; dir < 3; ++dir)
+            hipLaunchKernelGGL(lod_sweep_any, dim3(gblocks), dim3(256), 0, st, p, rate, E, F, side, bnd, dt, N, dir);
     }
     return hipGetLastError();
 }
@@ -450,11 +465,12 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
     const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
-    if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256)) {
+    if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256 || N == 512)) {
         float* const* tabs = ctx->ptab[s];
         if (N == 128) return launch_mixed<128>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
         if (N == 192) return launch_mixed<192>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
-        return launch_mixed<256>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        if (N == 256) return launch_mixed<256>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        return launch_mixed<512>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
     }
     switch (N) {
         case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, spill, st);
@@ -584,13 +600,13 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
                           ctx->tune[GELL_KNOB_LOD_YZ_LDS]};
     int best[3] = {1, 1, saved[2]};
     double best_t = 1e30;
-    const bool part_ok = (ctx->N == 128 || ctx->N == 192 || ctx->N == 256) && !bit_exact;
-    const int cand[4] = {1, 2, 4, 8};
+    const bool part_ok = (ctx->N == 128 || ctx->N == 192 || ctx->N == 256 || ctx->N == 512) && !bit_exact;
+    const int cand[5] = {1, 2, 4, 8, 16};
     ctx->field[0] = scratch;
     for (int xi = 0; xi < (part_ok ? 4 : 1) && e == hipSuccess; ++xi)
-        for (int yi = 0; yi < (part_ok ? 4 : 1) && e == hipSuccess; ++yi)
+        for (int yi = 0; yi < (part_ok ? 5 : 1) && e == hipSuccess; ++yi)
             for (int sp = 0; sp < 2 && e == hipSuccess; ++sp) {
-                if (cand[yi] > 1 && sp == 1) continue;  // the spill only exists for serial y/z
+                if ((cand[yi] > 1 || ctx->N > 256) && sp == 1) continue;  // the spill exists for serial y/z only
                 const int xp = cand[xi], yzp = cand[yi];
                 if ((xp > 1 && ctx->N % (xp * 32) != 0) || (yzp > 1 && ctx->N % (yzp * 32) != 0)) continue;
                 ctx->tune[GELL_KNOB_LOD_X_PARTS] = xp;
@@ -658,9 +674,9 @@ hipError_t lod_init_coeffs(gell_ctx* ctx, int s) {
     const double diag64 = corner64 - side64;
     // The host tables must outlive the asynchronous copies: they are kept
     // until the stream synchronisation below.
-    const int parts[3] = {2, 4, 8};
-    std::vector<float> tabs[3];
-    for (int q = 0; q < 3; ++q) {
+    const int parts[4] = {2, 4, 8, 16};
+    std::vector<float> tabs[4];
+    for (int q = 0; q < 4; ++q) {
         if (N % (parts[q] * kChunk) != 0) continue;
         tabs[q] = part_table(N, parts[q], side64, corner64, diag64);
         hipError_t e = hipMemcpyAsync(ctx->ptab[s][q], tabs[q].data(), tabs[q].size() * sizeof(float),

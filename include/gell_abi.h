@@ -173,15 +173,16 @@ int gell_sync(gell_ctx* ctx);
  *                              Default 3.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
- *                              to it), 2, 4 or 8 = each line split into
+ *                              to it), 2, 4, 8 or 16 = each line split into
  *                              blocks solved by separate waves and joined
  *                              through the exact reduced system (same linear
  *                              solve, rounding differs at the 1e-7 relative
- *                              level); applies for N in {128, 192, 256}, a
- *                              block count the grid cannot split into
+ *                              level); applies for N in {128, 192, 256, 512},
+ *                              a block count the grid cannot split into
  *                              32-voxel chunks falls back to a smaller one.
  *                              Default 4.
- *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep. Default 4.
+ *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep (at most 8).
+ *                              Default 4.
  * The first two knobs never change results; gell_autotune sets all four. */
 enum {
     GELL_KNOB_LOD_YZ_LDS = 0,

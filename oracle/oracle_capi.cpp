@@ -85,9 +85,38 @@ int32_t run(const gell_params* P, gell_cell* io_cells, int32_t n, void* io_field
     return n;
 }
 
+// Exactly `steps` mechanics steps (FM, uptake maps, diffusion substeps), no PD.
+template <class R>
+int32_t run_mech(const gell_params* P, gell_cell* io_cells, int32_t n, void* io_fields, int32_t steps) {
+    const int64_t V = (int64_t)P->N * P->N * P->N;
+    std::vector<CellT<R>> cells = load_cells<R>(io_cells, n);
+    std::vector<std::vector<R>> fields(P->n_substrates), rates(P->n_substrates), E(P->n_substrates),
+        F(P->n_substrates);
+    R* fio = (R*)io_fields;
+    for (int s = 0; s < P->n_substrates; ++s) {
+        fields[s].assign(fio + s * V, fio + (s + 1) * V);
+        rates[s].assign(V, R(0));
+        lod_coeffs<R>(P->N, (R)P->dx, (R)P->dt_diff, (R)P->sub[s].D, (R)P->sub[s].decay, E[s], F[s]);
+    }
+    const int32_t diff_steps = float_loop_count(P->dt_mech, P->dt_diff);
+    FMState<R> st;
+    for (int32_t m = 0; m < steps; ++m) mech_step<R>(P, cells, n, fields, rates, E, F, diff_steps, st, nullptr);
+    store_cells<R>(cells, io_cells, n);
+    for (int s = 0; s < P->n_substrates; ++s) std::copy(fields[s].begin(), fields[s].end(), fio + s * V);
+    return n;
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t orc_run_mech(const gell_params* P, gell_cell* cells, int32_t n, float* fields, int32_t steps) {
+    return run_mech<float>(P, cells, n, fields, steps);
+}
+
+int32_t orc_run_mech_f64(const gell_params* P, gell_cell* cells, int32_t n, double* fields, int32_t steps) {
+    return run_mech<double>(P, cells, n, fields, steps);
+}
 
 int orc_omp_threads(void) {
 #ifdef _OPENMP

@@ -60,6 +60,8 @@ def load(omp: bool = False) -> C.CDLL:
         "orc_death": (i32, [vp, i32]),
         "orc_run": (i32, [vp, vp, i32, vp, i32, i32, i32, u64, vp]),
         "orc_run_f64": (i32, [vp, vp, i32, vp, i32, i32, i32, u64]),
+        "orc_run_mech": (i32, [vp, vp, i32, vp, i32]),
+        "orc_run_mech_f64": (i32, [vp, vp, i32, vp, i32]),
     }
     for n, (res, args) in sig.items():
         f = getattr(lib, n)
@@ -189,3 +191,8 @@ class Oracle:
         t = times if times is not None else np.zeros(3, np.float64)
         return self.lib.orc_run(_pp(P), _p(cells), n, _p(fields), bio_steps, mech_limit, int(with_pd), first_step,
                                 _p(t))
+
+    def run_mech(self, P, cells: np.ndarray, n: int, fields: np.ndarray, steps: int) -> int:
+        """Exactly `steps` mechanics steps without PD; float or double fields."""
+        fn = self.lib.orc_run_mech_f64 if fields.dtype == np.float64 else self.lib.orc_run_mech
+        return fn(_pp(P), _p(cells), n, _p(fields), steps)

@@ -55,7 +55,8 @@ def test_bad_knob_is_an_error():
 
 
 @pytest.mark.parametrize("N,xp,yzp", [(128, 2, 2), (128, 4, 4), (192, 2, 2), (256, 2, 1), (256, 1, 2),
-                                      (256, 4, 4), (256, 2, 4)])
+                                      (256, 4, 4), (256, 2, 4), (256, 8, 16), (512, 4, 4), (512, 8, 8),
+                                      (512, 8, 16), (512, 1, 1)])
 def test_lod_partitioned_within_tolerance(N, xp, yzp, oracle):
     """Partitioned line solves reproduce the serial sweep within 1e-6
     relative L2 per substep (the stated bar for fields is 1e-5) and give the
