@@ -52,6 +52,12 @@ CONFIGS = {
 }
 
 
+# Batched ensemble: 64 replicates, n log-uniform in [1k, 200k], each in the
+# smallest grid of {64, 96, 128, 192}^3 (10 um voxels) that holds its ball
+# (density of the 1M-cell / 1.2 mm ball) with a 50 um margin; seed + r.
+ENSEMBLE = dict(replicates=64, n_lo=1_000, n_hi=200_000, grids=(64, 96, 128, 192), dx=10.0)
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -215,12 +221,125 @@ def only_leg(cfg, P, cells, what: str, steps: int, warmup: int) -> dict:
     return out
 
 
+def ensemble_workload(seed: int):
+    from gell_amd import runtime as rt
+    from gell_amd import spheroid
+
+    E = ENSEMBLE
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(E["replicates"]):
+        n = int(round(np.exp(rng.uniform(np.log(E["n_lo"]), np.log(E["n_hi"])))))
+        radius = 1200.0 * (n / 1e6) ** (1 / 3)
+        N = next(g for g in E["grids"] if g * E["dx"] >= 2 * radius + 100.0)
+        P = rt.make_params(N=N, dx=E["dx"], capacity=n + n // 4 + 64, seed=seed + r, substrates=SUBSTRATES[:1])
+        cells = spheroid.make_cells(n, N, E["dx"], radius, seed + r, capacity=P.capacity)
+        reps.append((P, cells, n))
+    return reps
+
+
+def ensemble_main(args) -> int:
+    """Config B: the whole ensemble per biology step, graph mode, against the
+    same replicates run one after another through the per-context calls."""
+    from gell_amd import runtime as rt
+
+    reps = ensemble_workload(args.seed)
+    mech = rt.float_loop_count(6.0, 0.1)
+    diff = rt.float_loop_count(0.1, 0.01)
+    ens = rt.Ensemble([P for P, _, _ in reps])
+    ens.autotune()
+    bufs = []
+    for P, cells, _ in reps:
+        b = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
+        b.upload(cells)
+        bufs.append(b)
+    n0 = [n for _, _, n in reps]
+
+    def run(mode, steps, warmup):
+        ens.set_mode(mode)
+        for b, (_, cells, _) in zip(bufs, reps):
+            b.upload(cells)
+        n = list(n0)
+        for w in range(warmup):
+            n = ens.bio_step(bufs, n, w)
+        ens.sync()
+        agents = 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            agents += sum(n) * mech
+            n = ens.bio_step(bufs, n, warmup + k)
+        ens.sync()
+        return agents, time.perf_counter() - t0, n
+
+    agents, secs, n_end = run(rt.Ensemble.GRAPH, args.steps, args.warmup)
+    log(f"[bench B] graph: {secs:.3f} s for {args.steps} steps")
+    a_st, s_st, _ = run(rt.Ensemble.STREAMS, 1, 0)
+    log(f"[bench B] streams: {s_st:.3f} s for 1 step")
+    stats = ens.stats()
+    ens.close()
+
+    # the same replicates one after another, each through the per-context calls
+    seq_agents, seq_secs = 0, 0.0
+    for (P, cells, n), b in zip(reps, bufs):
+        ctx = rt.Context(P)
+        ctx.autotune()
+        b.upload(cells)
+        ctx.sync()
+        t0 = time.perf_counter()
+        seq_agents += n * mech
+        n = ctx.birth(b, n, 0)
+        for _ in range(mech):
+            ctx.fm_update(b, n)
+            ctx.prelod(0, b, n)
+            for _ in range(diff):
+                ctx.lod_substep(0)
+        n = ctx.death(b, n)
+        ctx.sync()
+        seq_secs += time.perf_counter() - t0
+        ctx.close()
+    for b in bufs:
+        b.free()
+    log(f"[bench B] sequential: {seq_secs:.3f} s for 1 step")
+    grids = {}
+    for P, _, _ in reps:
+        grids[P.N] = grids.get(P.N, 0) + 1
+    line = {
+        "metric": METRIC,
+        "value": agents / secs,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded spheroids)",
+        "config": {
+            "workload": (f"B: {len(reps)} replicates, n log-uniform in [{ENSEMBLE['n_lo']}, {ENSEMBLE['n_hi']}], "
+                         f"grids {grids} at 10 um, oxygen; step = one biology step of every replicate"),
+            "cells_total": sum(n0),
+            "voxels_total": sum(P.N ** 3 for P, _, _ in reps),
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+        "streams_value": a_st / s_st,
+        "sequential_value": seq_agents / seq_secs,
+        "speedup_vs_sequential": (agents / secs) / (seq_agents / seq_secs),
+        "graph_stats": stats,
+        "n_final_total": sum(n_end),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=2, help="timed biology steps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed biology steps")
-    ap.add_argument("--config", choices=sorted(CONFIGS), default="L")
+    ap.add_argument("--config", choices=sorted(CONFIGS) + ["B"], default="L")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
     ap.add_argument("--only", choices=["lod", "fm", "copy"], default=None,
@@ -233,6 +352,8 @@ def main() -> int:
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
 
+    if args.config == "B":
+        return ensemble_main(args)
     cfg = CONFIGS[args.config]
     log(f"[bench] building workload {args.config}")
     P, cells = make_workload(cfg, args.seed)

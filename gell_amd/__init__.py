@@ -10,6 +10,7 @@ from .runtime import (  # noqa: F401
     CELL_DTYPE,
     Context,
     DeviceBuffer,
+    Ensemble,
     GellError,
     Params,
     default_params,
@@ -22,6 +23,7 @@ __all__ = [
     "CELL_DTYPE",
     "Context",
     "DeviceBuffer",
+    "Ensemble",
     "GellError",
     "Params",
     "default_params",

@@ -359,6 +359,12 @@ int gell_sync(gell_ctx* ctx) {
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "sync");
 }
 
+int32_t gell_float_loop_count(float end, float step) {
+    int32_t k = 0;
+    for (volatile float t = 0.0f; t < end; t = t + step) ++k;
+    return k;
+}
+
 int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");

@@ -88,6 +88,13 @@ hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n
 
 hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out);
 hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out);
+// Asynchronous halves of the two: *_begin enqueues the kernels and the count
+// read-back on the context stream; *_finish waits for the stream and returns
+// the new count.
+hipError_t birth_begin(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step);
+hipError_t birth_finish(gell_ctx* ctx, int32_t n, int32_t* n_out);
+hipError_t death_begin(gell_ctx* ctx, gell_cell* cells, int32_t n);
+hipError_t death_finish(gell_ctx* ctx, int32_t n, int32_t* n_out);
 
 // Radix-sort planning: number of passes and bits per pass for a key width.
 struct SortPlan {

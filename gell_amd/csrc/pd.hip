@@ -8,6 +8,8 @@
 // mothers in slot order (wave ballots + a small scan of per-block counts);
 // divisions beyond capacity - 1 are dropped in that same order.
 // Death: stable stream compaction of live records by the same scan.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -199,17 +201,25 @@ __global__ __launch_bounds__(kBlock) void death_scatter(const gell_cell* __restr
     if (live) store_cell(out, rank, load_cell(cells, i));
 }
 
+// Copies the first *count records of src to dst (count read on the device, so
+// the compaction needs no host round trip before its copy-back). Grid-stride
+// over 8-byte words; the grid is sized for the upper bound n.
+__global__ __launch_bounds__(kBlock) void copy_counted(const uint2* __restrict__ src, uint2* __restrict__ dst,
+                                                       const int32_t* __restrict__ count) {
+    const int64_t words = (int64_t)(*count) * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
-hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out) {
-    *n_out = n;
+hipError_t birth_begin(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step) {
+    hipStream_t st = ctx->stream;
     if (n <= 0 || n >= ctx->P.capacity) {
         // PDkernel.cuh:31: nothing happens once the array is full.
-        hipError_t e = hipMemsetAsync(ctx->slots, 0xFF, sizeof(int32_t) * (size_t)(n > 0 ? n : 0), ctx->stream);
-        if (e != hipSuccess) return e;
-        return hipStreamSynchronize(ctx->stream);
+        if (n > 0) return hipMemsetAsync(ctx->slots, 0xFF, sizeof(int32_t) * (size_t)n, st);
+        return hipSuccess;
     }
-    hipStream_t st = ctx->stream;
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     const uint32_t k0 = (uint32_t)ctx->P.seed, k1 = (uint32_t)(ctx->P.seed >> 32);
     const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
@@ -220,39 +230,53 @@ hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t ste
                        ctx->blk, k0, k1, s0, s1, ctx->slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+}
+
+hipError_t birth_finish(gell_ctx* ctx, int32_t n, int32_t* n_out) {
+    *n_out = n;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess || n <= 0 || n >= ctx->P.capacity) return e;
     const int64_t room = (int64_t)ctx->P.capacity - 1 - n;
     const int64_t births = *ctx->h_total < room ? *ctx->h_total : (room > 0 ? room : 0);
     *n_out = (int32_t)(n + births);
     return hipSuccess;
 }
 
-hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out) {
+hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out) {
     *n_out = n;
+    hipError_t e = birth_begin(ctx, cells, n, step);
+    return e != hipSuccess ? e : birth_finish(ctx, n, n_out);
+}
+
+hipError_t death_begin(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     if (n <= 0) return hipSuccess;
     hipStream_t st = ctx->stream;
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk);
     hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanThreads), 0, st, ctx->blk, (int32_t)blocks, ctx->d_total);
     hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk, ctx->compact);
+    const int64_t words = (int64_t)n * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
+    const int64_t cblocks = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(copy_counted, dim3((unsigned)cblocks), dim3(kBlock), 0, st,
+                       reinterpret_cast<const uint2*>(ctx->compact), reinterpret_cast<uint2*>(cells), ctx->d_total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    const int32_t live = *ctx->h_total;
-    if (live > 0) {
-        e = hipMemcpyAsync(cells, ctx->compact, sizeof(gell_cell) * (size_t)live, hipMemcpyDeviceToDevice, st);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return e;
-    }
-    *n_out = live;
+    return hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+}
+
+hipError_t death_finish(gell_ctx* ctx, int32_t n, int32_t* n_out) {
+    *n_out = n;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess || n <= 0) return e;
+    *n_out = *ctx->h_total;
     return hipSuccess;
+}
+
+hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out) {
+    *n_out = n;
+    hipError_t e = death_begin(ctx, cells, n);
+    return e != hipSuccess ? e : death_finish(ctx, n, n_out);
 }
 
 }  // namespace gell

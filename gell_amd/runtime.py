@@ -122,6 +122,18 @@ def load_library() -> C.CDLL:
         "gell_device_sync": (C.c_int, []),
         "gell_stream_copy": (C.c_int, [vp, vp, sz, vp]),
         "gell_device_name": (C.c_int, [C.c_char_p, C.c_int]),
+        "gell_float_loop_count": (i32, [C.c_float, C.c_float]),
+        "gell_ensemble_create": (C.c_int, [P(Params), i32, vp, P(vp)]),
+        "gell_ensemble_destroy": (None, [vp]),
+        "gell_ensemble_size": (i32, [vp]),
+        "gell_ensemble_member": (vp, [vp, i32]),
+        "gell_ensemble_last_error": (C.c_char_p, [vp]),
+        "gell_ensemble_set_mode": (C.c_int, [vp, i32]),
+        "gell_ensemble_autotune": (C.c_int, [vp, C.c_int]),
+        "gell_ensemble_mech_steps": (C.c_int, [vp, P(vp), P(i32), i32]),
+        "gell_ensemble_bio_step": (C.c_int, [vp, P(vp), P(i32), u64]),
+        "gell_ensemble_sync": (C.c_int, [vp]),
+        "gell_ensemble_stats": (C.c_int, [vp, P(i64), P(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -346,6 +358,88 @@ class Context:
         calls = (C.c_int64 * N_PHASES)()
         self._check(self.lib.gell_profile_read(self.h, ms, calls, int(reset)), "gell_profile_read")
         return {PHASE_NAMES[q]: (ms[q], calls[q]) for q in range(N_PHASES)}
+
+
+class _Member(Context):
+    """A context owned by an Ensemble (not destroyed on its own)."""
+
+    def __init__(self, handle: int, params: Params):
+        self.lib = load_library()
+        self.params = params
+        self.h = handle
+        self.N = params.N
+        self.V = params.N ** 3
+
+    def close(self) -> None:
+        self.h = None
+
+
+class Ensemble:
+    """Independent replicates advanced together (gell_ensemble_* of the C
+    ABI): one context per replicate on its own stream, mechanics steps
+    replayed as one hipGraph per set of cell counts."""
+
+    STREAMS, GRAPH = 0, 1
+
+    def __init__(self, params: list[Params], stream: int | None = None):
+        self.lib = load_library()
+        self.params = list(params)
+        arr = (Params * len(params))(*params)
+        h = C.c_void_p()
+        st = self.lib.gell_ensemble_create(arr, len(params), stream, C.byref(h))
+        if st != 0:
+            raise GellError(f"gell_ensemble_create: {self.lib.gell_ensemble_last_error(None).decode()} ({st})")
+        self.h = h.value
+        self.members = [_Member(self.lib.gell_ensemble_member(self.h, r), p) for r, p in enumerate(params)]
+
+    def __len__(self) -> int:
+        return len(self.members)
+
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            raise GellError(f"{what}: {self.lib.gell_ensemble_last_error(self.h).decode()} ({st})")
+
+    def set_mode(self, mode: int) -> None:
+        self._check(self.lib.gell_ensemble_set_mode(self.h, mode), "gell_ensemble_set_mode")
+
+    def autotune(self, bit_exact: bool = False) -> None:
+        self._check(self.lib.gell_ensemble_autotune(self.h, 1 if bit_exact else 0), "gell_ensemble_autotune")
+
+    @staticmethod
+    def _arrays(cells, n):
+        ptrs = (C.c_void_p * len(cells))(*[_ptr(c) for c in cells])
+        counts = (C.c_int32 * len(n))(*[int(x) for x in n])
+        return ptrs, counts
+
+    def mech_steps(self, cells, n, steps: int) -> None:
+        ptrs, counts = self._arrays(cells, n)
+        self._check(self.lib.gell_ensemble_mech_steps(self.h, ptrs, counts, steps), "gell_ensemble_mech_steps")
+
+    def bio_step(self, cells, n, step: int) -> list[int]:
+        ptrs, counts = self._arrays(cells, n)
+        self._check(self.lib.gell_ensemble_bio_step(self.h, ptrs, counts, step), "gell_ensemble_bio_step")
+        return list(counts)
+
+    def sync(self) -> None:
+        self._check(self.lib.gell_ensemble_sync(self.h), "gell_ensemble_sync")
+
+    def stats(self) -> dict:
+        cap, lau = C.c_int64(), C.c_int64()
+        self._check(self.lib.gell_ensemble_stats(self.h, C.byref(cap), C.byref(lau)), "gell_ensemble_stats")
+        return {"graph_captures": cap.value, "graph_launches": lau.value}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.close()
+            self.lib.gell_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _ptr(x) -> int:

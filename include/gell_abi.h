@@ -226,6 +226,51 @@ int gell_device_sync(void);
 int gell_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 int gell_device_name(char* buf, int len);
 
+/* Trip count of the reference's float-accumulated time loops
+ * `for (float t = 0; t < end; t += step)` (Main.cu:108,114,121): 61 for
+ * (6, 0.1f), 11 for (0.1f, 0.01f). */
+int32_t gell_float_loop_count(float end, float step);
+
+/* ---------------------------------------------------------------------------
+ * Ensembles: many independent replicates (each its own N, dx, capacity,
+ * seed and substrates; shared dt_bio / dt_mech / dt_diff) advanced together.
+ * The reference runs one simulation per process (Main.cu:61-127); this is
+ * its batched form. Replicate r is an ordinary context on a stream of its
+ * own, so every per-context call (gell_field, gell_tune, ...) applies to
+ * gell_ensemble_member(ens, r), and its results are bit-identical to
+ * running that replicate alone through the per-context calls.
+ * ------------------------------------------------------------------------- */
+typedef struct gell_ensemble gell_ensemble;
+enum {
+    GELL_ENSEMBLE_STREAMS = 0, /* each mechanics step forked onto the replicate streams */
+    GELL_ENSEMBLE_GRAPH = 1    /* the same fork/join step replayed as one hipGraph (default) */
+};
+/* params[0..count); `stream` (a hipStream_t) orders the ensemble's steps
+ * against the caller's work; with NULL the ensemble makes a stream of its
+ * own (the null stream cannot be captured into a graph). */
+int gell_ensemble_create(const gell_params* params, int32_t count, void* stream, gell_ensemble** out);
+void gell_ensemble_destroy(gell_ensemble* ens);
+int32_t gell_ensemble_size(const gell_ensemble* ens);
+gell_ctx* gell_ensemble_member(gell_ensemble* ens, int32_t r);
+const char* gell_ensemble_last_error(const gell_ensemble* ens);
+int gell_ensemble_set_mode(gell_ensemble* ens, int32_t mode);
+/* gell_autotune once per distinct grid size; replicates of that size share
+ * the result. */
+int gell_ensemble_autotune(gell_ensemble* ens, int flags);
+/* `steps` mechanics steps of every replicate with fixed counts: FM update,
+ * uptake map of every substrate, then each substrate's LOD substeps
+ * (Main.cu:114-124). cells[r] holds params[r].capacity records, n[r] the
+ * live count. In graph mode the first step after a count change is issued
+ * directly and the following ones replay a captured graph. Asynchronous. */
+int gell_ensemble_mech_steps(gell_ensemble* ens, gell_cell* const* cells, const int32_t* n, int32_t steps);
+/* One biology step of every replicate (Main.cu:108-127): birth with
+ * Philox draws keyed by (params[r].seed, slot, bio_step), the mechanics
+ * steps, death; n[r] is updated in place. Synchronises. */
+int gell_ensemble_bio_step(gell_ensemble* ens, gell_cell* const* cells, int32_t* n, uint64_t bio_step);
+int gell_ensemble_sync(gell_ensemble* ens);
+/* Number of graph captures and graph launches so far. */
+int gell_ensemble_stats(const gell_ensemble* ens, int64_t* captures, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif
