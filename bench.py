@@ -34,7 +34,12 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "agent-steps/s & voxel-updates/s, 1M cells/256^3; surrogate vs LOD at 1e-3 L2"
 HBM_PEAK_GBS = 8000.0
-LOD_BYTES_PER_VOXEL = 28  # x sweep 12 (p, p_rate in, p out) + y 8 + z 8
+# Algorithmic HBM bytes per voxel-update of one LOD substep: three sweeps
+# x 12 (p, p_rate in; p out) + y 8 + z 8 = 28; the fused substep (x sweep and
+# y blocks in one pass, y coupling applied inside the z sweep) moves
+# 12 + 8 = 20.
+LOD_BYTES_PER_VOXEL = 28
+LOD_BYTES_PER_VOXEL_FUSED = 20
 
 # Substrate constants: oxygen as in the reference's Definitions.h:21-25;
 # glucose with a smaller diffusivity, no decay and its own boundary value.
@@ -179,7 +184,7 @@ def cpu_leg(cfg, P, cells, budget_s):
     )
 
 
-def only_leg(cfg, P, cells, what: str, steps: int, warmup: int) -> dict:
+def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | None = None) -> dict:
     """One kernel group alone, for profiler runs: `lod` (LOD substeps of
     substrate 0 after one mechanics step + uptake map), `fm` (mechanics steps)
     or `copy` (the streaming copy kernel over 256 MiB, the byte-counter
@@ -188,6 +193,8 @@ def only_leg(cfg, P, cells, what: str, steps: int, warmup: int) -> dict:
 
     n = cfg["cells"]
     ctx = rt.Context(P)
+    for k, v in (knobs or {}).items():
+        ctx.tune(k, v)
     buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
     buf.upload(cells)
     ctx.fm_update(buf, n)
@@ -344,6 +351,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
     ap.add_argument("--only", choices=["lod", "fm", "copy"], default=None,
                     help="run one kernel group alone (profiler runs); prints its timings")
+    ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
+                    help="kernel-variant knob for --only runs (e.g. lod_parts=8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -358,7 +367,8 @@ def main() -> int:
     log(f"[bench] building workload {args.config}")
     P, cells = make_workload(cfg, args.seed)
     if args.only:
-        print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup)), flush=True)
+        knobs = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.knob)}
+        print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup, knobs)), flush=True)
         return 0
     g = gpu_leg(cfg, P, cells, args.steps, args.warmup)
 
@@ -367,7 +377,10 @@ def main() -> int:
     lod_subs = args.steps * g["mech"] * g["diff"] * P.n_substrates
     lod_ms, lod_calls = g["prof"]["lod"]
     lod_us = lod_ms / max(lod_calls, 1) * 1e3
-    achieved = LOD_BYTES_PER_VOXEL * V / (lod_us * 1e-6) / 1e9
+    tuned = g["tuned"]
+    fused = bool(tuned.get("lod_fused")) and tuned["lod_parts"] > 1 and cfg["N"] in (128, 256, 512)
+    bpv = LOD_BYTES_PER_VOXEL_FUSED if fused else LOD_BYTES_PER_VOXEL
+    achieved = bpv * V / (lod_us * 1e-6) / 1e9
     traffic = read_traffic(f"lod_substep_{cfg['N']}_traffic.txt")
     phases = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in g["prof"].items() if v[1]}
 
@@ -413,13 +426,15 @@ def main() -> int:
         "voxel_updates_per_s": V * lod_subs / secs,
         "roofline": {
             "bound": "hbm",
-            "kernel": "LOD substep (x sweep with boundary + uptake, y sweep, z sweep)",
+            "kernel": ("LOD substep, fused: x sweep + y blocks (with boundary + uptake), y reduce, "
+                       "z sweep with y correction" if fused else
+                       "LOD substep (x sweep with boundary + uptake, y sweep, z sweep)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_per_launch": LOD_BYTES_PER_VOXEL * V,
+            "bytes_per_launch": bpv * V,
             "us_per_launch": lod_us,
         },
         "cpu_baseline": cpu,

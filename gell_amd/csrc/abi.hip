@@ -113,6 +113,7 @@ void release(gell_ctx* c) {
         dfree(c->acc[s]);
         dfree(c->rate_list[s]);
     }
+    dfree(c->coup);
     dfree(c->flags);
     dfree(c->blk);
     dfree(c->slots);
@@ -223,6 +224,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->slots, cap));
     A(dalloc(&c->d_total, 1));
     A(dalloc(&c->compact, cap));
+    if (P.N == 128 || P.N == 256 || P.N == 512) A(dalloc(&c->coup, (size_t)(4 * (P.N / 64)) * (size_t)P.N * P.N));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_total), sizeof(int32_t));
     if (e != hipSuccess) {
         release(c);

@@ -54,7 +54,8 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 3, 4, 4};
+    int tune[GELL_N_KNOBS] = {1, 3, 4, 4, 1};
+    float* coup = nullptr;  // y-coupling values of the fused LOD substep (4 (N / 64) N^2 floats)
 
     // Profiling.
     bool prof = false;
@@ -74,6 +75,8 @@ enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, P
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s);
+// Whether the next substep takes the fused path (knobs and grid allow it).
+bool lod_fused_active(const gell_ctx* ctx);
 hipError_t lod_init_coeffs(gell_ctx* ctx, int s);
 hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us);
 

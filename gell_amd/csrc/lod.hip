@@ -176,7 +176,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPILL > 0 ? 
 //   rows 1..   [side, diag, side]        d = p
 //   row N-1    [0 ... 0, 1]              d = b  (p[N-1] = b)
 // and x[0] overwritten with b afterwards. Splitting the rows into P blocks of
-// m = N / P, each block is solved alone with its own Thomas factors (El, Fl),
+// m = N / P, each block is solved alone with its own Thomas factors (El, Fl;
+// the table holds iFl = 1 / Fl, so the back sweep multiplies),
 // giving y_k; the exact solution is x_k = y_k - u_{k-1} W_k - t_{k+1} V_k,
 // where W_k, V_k are the block's spikes (its solution for the coupling
 // coefficient placed on its first / last row) and t_k, u_k are the first and
@@ -184,23 +185,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPILL > 0 ? 
 // the inverse of the reduced 2P x 2P system. Spikes and the reduced inverse
 // depend only on the matrix, which every line shares, so they are computed
 // once on the host (in double) and read as wave-uniform scalars.
-// Table layout (floats): El[N] | Fl[N] | W[N] | V[N] | Rinv[(2P)^2].
+// Table layout (floats): El[N] | iFl[N] | W[N] | V[N] | Rinv[(2P)^2].
 // ---------------------------------------------------------------------------
+// Block-local forward / backward sweep with the block's own Thomas factors
+// (iFl = reciprocal pivots).
+template <int m>
+__device__ __forceinline__ void part_local(float (&v)[m], const float* __restrict__ El, const float* __restrict__ iFl,
+                                           float side) {
+#pragma unroll
+    for (int i = 1; i < m; ++i) v[i] = v[i] - El[i] * v[i - 1];
+    v[m - 1] = v[m - 1] * iFl[m - 1];
+#pragma unroll
+    for (int i = m - 2; i >= 0; --i) v[i] = (v[i] - side * v[i + 1]) * iFl[i];
+}
+
 template <int N, int P>
 __device__ __forceinline__ void part_solve(float (&v)[N / P], const float* __restrict__ tab, int k, float side,
                                            float bnd, float* __restrict__ bt, float* __restrict__ bb, int lane) {
     constexpr int m = N / P;
-    const float* El = tab + k * m;
-    const float* Fl = tab + N + k * m;
     const float* W = tab + 2 * N + k * m;
     const float* Vs = tab + 3 * N + k * m;
     const float* Rinv = tab + 4 * N;
-    // local forward / backward sweep
-#pragma unroll
-    for (int i = 1; i < m; ++i) v[i] = v[i] - El[i] * v[i - 1];
-    v[m - 1] = v[m - 1] / Fl[m - 1];
-#pragma unroll
-    for (int i = m - 2; i >= 0; --i) v[i] = (v[i] - side * v[i + 1]) / Fl[i];
+    part_local<m>(v, tab + k * m, tab + N + k * m, side);
     // exchange block boundary values through LDS
     bt[k * 64 + lane] = v[0];
     bb[k * 64 + lane] = v[m - 1];
@@ -256,29 +262,26 @@ __global__ __launch_bounds__(64 * P) void lod_yz_part(float* __restrict__ p, con
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[i]), rs, voff, so0 + i * stride_b, 0);
 }
 
-// x sweep with fused boundary + uptake, P waves per 64 lines; each wave
-// loads its m-column segment through the LDS transpose tile.
+// x sweep with fused boundary + uptake for the 64 x lines L0.. (P waves per
+// line, wave k owning columns [k m, (k + 1) m)); each wave loads its segment
+// through its LDS transpose tile and leaves its solved segment in v.
 template <int N, int P>
-__global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, const float* __restrict__ rate,
-                                                     const float* __restrict__ tab, float side, float bnd, float dt) {
+__device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __restrict__ p,
+                                              const float* __restrict__ rate, const float* __restrict__ tab,
+                                              float side, float bnd, float dt, int64_t L0, int k, int lane,
+                                              float* __restrict__ tile, float* __restrict__ bt,
+                                              float* __restrict__ bb) {
     constexpr int m = N / P;
     static_assert(m % kChunk == 0, "segment must be a whole number of chunks");
     constexpr int NC = m / kChunk;
     constexpr int Q = kChunk / 4;
     constexpr int LPI = 64 / Q;
     constexpr int64_t NN = (int64_t)N * N;
-    __shared__ float tiles[P][64 * kTileLd];
-    __shared__ float bt[P * 64], bb[P * 64];
-    const int lane = threadIdx.x & 63;
-    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* tile = tiles[k];
-    const int64_t L0 = (int64_t)blockIdx.x * 64;
     const __amdgpu_buffer_rsrc_t rp = field_rsrc(p, NN * N);
     const __amdgpu_buffer_rsrc_t rr = field_rsrc(rate, NN * N);
     const int lq = lane / Q, part = lane % Q;
     const int voff = (lq * N + part * 4) * 4;
     const int sbase = (int)(L0 * N * 4) + k * m * 4;
-    float v[m];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -308,6 +311,24 @@ __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, cons
     if (k == 0) v[0] = bnd;
     if (k == P - 1) v[m - 1] = bnd;
     part_solve<N, P>(v, tab, k, side, bnd, bt, bb, lane);
+}
+
+template <int N, int P>
+__global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, const float* __restrict__ rate,
+                                                     const float* __restrict__ tab, float side, float bnd, float dt) {
+    constexpr int m = N / P;
+    constexpr int NC = m / kChunk;
+    constexpr int Q = kChunk / 4;
+    constexpr int LPI = 64 / Q;
+    __shared__ float tiles[P][64 * kTileLd];
+    __shared__ float bt[P * 64], bb[P * 64];
+    const int lane = threadIdx.x & 63;
+    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* tile = tiles[k];
+    const int64_t L0 = (int64_t)blockIdx.x * 64;
+    const int lq = lane / Q, part = lane % Q;
+    float v[m];
+    x_lines_solve<N, P>(v, p, rate, tab, side, bnd, dt, L0, k, lane, tile, bt, bb);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -324,6 +345,143 @@ __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, cons
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused substep for N = 64 P (P = 2, 4, 8): 20 bytes of HBM traffic per voxel
+// instead of 28, with the same arithmetic as lod_x_part<N, P> followed by
+// lod_yz_part<N, 1, P> and lod_yz_part<N, 2, PZ> (bit-identical results).
+//
+//   lod_xy_part    one workgroup per (z plane, block of 64 y rows): the x
+//                  sweep of those 64 lines (P waves, as lod_x_part), the block
+//                  into LDS, then one thread per column runs the y block's
+//                  local sweep; writes the local y solution to the field and
+//                  its first / last values to the coupling buffer.
+//   lod_y_reduce   per y line: the P blocks' first / last values through the
+//                  reduced inverse -> the two coupling coefficients per block.
+//   lod_z_part_ycorr  z sweep (PZ waves per 64 lines) whose loads apply the
+//                  y correction x = y - um1 W[j] - tp1 V[j] of their row.
+//
+// Coupling buffer (floats, 4 P N^2): first | last | um1 | tp1, each indexed
+// [(k P + block) N + i] for the y line (i, k).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const float* __restrict__ rate,
+                                                 const float* __restrict__ tab, float side, float bnd, float dt,
+                                                 float* __restrict__ coup) {
+    constexpr int P = N / 64;
+    constexpr int m = 64;  // x segment per wave = y rows per block
+    constexpr int64_t NN = (int64_t)N * N;
+    constexpr int RowLd = N + 1;
+    constexpr int XT = P * 64 * kTileLd;
+    constexpr int BT = 64 * RowLd;
+    __shared__ float smem[XT > BT ? XT : BT];
+    __shared__ float bt[P * 64], bb[P * 64];
+    const int lane = threadIdx.x & 63;
+    const int kx = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t L0 = (int64_t)blockIdx.x * 64;  // x lines (j, k), j = L % N
+    const int kz = (int)(L0 / N), yb = (int)((L0 % N) / 64);
+    float v[m];
+    x_lines_solve<N, P>(v, p, rate, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
+    // every wave is past its transpose-tile reads (part_solve synchronises)
+#pragma unroll
+    for (int t = 0; t < m; ++t) smem[lane * RowLd + kx * m + t] = v[t];
+    __syncthreads();
+    const int i = threadIdx.x;
+#pragma unroll
+    for (int t = 0; t < m; ++t) v[t] = smem[t * RowLd + i];
+    if (yb == 0) v[0] = bnd;
+    if (yb == P - 1) v[m - 1] = bnd;
+    part_local<m>(v, tab + yb * m, tab + N + yb * m, side);
+    coup[(kz * P + yb) * N + i] = v[0];
+    coup[P * NN + (kz * P + yb) * N + i] = v[m - 1];
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(p, NN * N);
+    const int so0 = (int)(L0 * N * 4);
+#pragma unroll
+    for (int t = 0; t < m; ++t)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[t]), rs, i * 4, so0 + t * N * 4, 0);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void lod_y_reduce(const float* __restrict__ tab, float* __restrict__ coup) {
+    constexpr int P = N / 64;
+    constexpr int64_t NN = (int64_t)N * N;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= NN) return;
+    const int kz = (int)(t / N), i = (int)(t % N);
+    const float* Rinv = tab + 4 * N;
+    float f[P], l[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        f[q] = coup[(kz * P + q) * N + i];
+        l[q] = coup[P * NN + (kz * P + q) * N + i];
+    }
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+        float um1 = 0.f, tp1 = 0.f;
+        if (b > 0) {
+            const float* row = Rinv + (2 * (b - 1) + 1) * (2 * P);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                um1 += row[2 * q] * f[q];
+                um1 += row[2 * q + 1] * l[q];
+            }
+        }
+        if (b < P - 1) {
+            const float* row = Rinv + (2 * (b + 1)) * (2 * P);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                tp1 += row[2 * q] * f[q];
+                tp1 += row[2 * q + 1] * l[q];
+            }
+        }
+        coup[2 * P * NN + (kz * P + b) * N + i] = um1;
+        coup[3 * P * NN + (kz * P + b) * N + i] = tp1;
+    }
+}
+
+template <int N, int PZ>
+__global__ __launch_bounds__(64 * PZ) void lod_z_part_ycorr(float* __restrict__ p, const float* __restrict__ tabz,
+                                                            const float* __restrict__ taby,
+                                                            const float* __restrict__ coup, float side, float bnd) {
+    constexpr int PY = N / 64;
+    constexpr int64_t NN = (int64_t)N * N;
+    constexpr int m = N / PZ;
+    __shared__ float bt[PZ * 64], bb[PZ * 64];
+    const int lane = threadIdx.x & 63;
+    const int kb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t L = (int64_t)blockIdx.x * 64 + lane;  // z line (i, j)
+    const int j = __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * 64) / N));
+    const int i = (int)(L % N);
+    const int yb = j / 64;
+    const float Wj = taby[2 * N + j], Vj = taby[3 * N + j];
+    // coupling pair of row block yb for the y line (i, k): um1 at
+    // [2 PY N^2 + (k PY + yb) N + i], tp1 PY N^2 floats further
+    const __amdgpu_buffer_rsrc_t rc = field_rsrc(coup, 4 * PY * NN);
+    const int coff = (int)((2 * PY * NN + yb * N + i) * 4);
+    constexpr int ct_b = (int)(PY * NN * 4);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(p, NN * N);
+    const int voff = (int)(L * 4);
+    constexpr int stride_b = (int)(NN * 4);
+    const int so0 = kb * m * stride_b;
+    float v[m];
+#pragma unroll
+    for (int t = 0; t < m; ++t) {
+        const int kz = kb * m + t;
+        const float y = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, so0 + t * stride_b, 0));
+        const int cso = kz * PY * N * 4;
+        const float um1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rc, coff, cso, 0));
+        const float tp1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rc, coff, cso + ct_b, 0));
+        float x = y - um1 * Wj;
+        x = x - tp1 * Vj;
+        v[t] = j == 0 ? bnd : x;
+    }
+    if (kb == 0) v[0] = bnd;
+    if (kb == PZ - 1) v[m - 1] = bnd;
+    part_solve<N, PZ>(v, tabz, kb, side, bnd, bt, bb, lane);
+#pragma unroll
+    for (int t = 0; t < m; ++t)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[t]), rs, voff, so0 + t * stride_b, 0);
 }
 
 // Any N, any direction (dir 0 = x with fused boundary + uptake).
@@ -452,7 +610,44 @@ hipError_t launch_mixed(float* p, const float* rate, const float* E, const float
     return hipGetLastError();
 }
 
+template <int N, int PZ>
+void launch_z_ycorr(float* p, const float* tabz, const float* taby, const float* coup, float side, float bnd,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((lod_z_part_ycorr<N, PZ>), dim3((unsigned)((int64_t)N * N / 64)), dim3(64 * PZ), 0, st, p,
+                       tabz, taby, coup, side, bnd);
+}
+
+// Fused substep (see lod_xy_part); zp = z blocks, falling back to the next
+// smaller count the grid supports.
+template <int N>
+hipError_t launch_fused(float* p, const float* rate, float* const* tabs, float side, float bnd, float dt, int zp,
+                        float* coup, hipStream_t st) {
+    constexpr int P = N / 64;
+    static_assert(P == 2 || P == 4 || P == 8, "fused substep needs N in {128, 256, 512}");
+    const float* taby = tabs[P == 2 ? 0 : (P == 4 ? 1 : 2)];
+    hipLaunchKernelGGL((lod_xy_part<N>), dim3((unsigned)((int64_t)N * N / 64)), dim3(N), 0, st, p, rate, taby, side,
+                       bnd, dt, coup);
+    hipLaunchKernelGGL((lod_y_reduce<N>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0, st, taby,
+                       coup);
+    constexpr bool ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0, ok16 = N % (16 * kChunk) == 0;
+    if (zp >= 16 && ok16)
+        launch_z_ycorr<N, ok16 ? 16 : 2>(p, tabs[3], taby, coup, side, bnd, st);
+    else if (zp >= 8 && ok8)
+        launch_z_ycorr<N, ok8 ? 8 : 2>(p, tabs[2], taby, coup, side, bnd, st);
+    else if (zp >= 4 && ok4)
+        launch_z_ycorr<N, ok4 ? 4 : 2>(p, tabs[1], taby, coup, side, bnd, st);
+    else
+        launch_z_ycorr<N, 2>(p, tabs[0], taby, coup, side, bnd, st);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+bool lod_fused_active(const gell_ctx* ctx) {
+    const int N = ctx->N;
+    return ctx->tune[GELL_KNOB_LOD_FUSED] && ctx->tune[GELL_KNOB_LOD_PARTS] > 1 && ctx->coup &&
+           (N == 128 || N == 256 || N == 512);
+}
 
 hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     const int N = ctx->N;
@@ -465,6 +660,12 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
     const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
     const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
+    if (lod_fused_active(ctx)) {
+        float* const* tabs = ctx->ptab[s];
+        if (N == 128) return launch_fused<128>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
+        if (N == 256) return launch_fused<256>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
+        return launch_fused<512>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
+    }
     if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256 || N == 512)) {
         float* const* tabs = ctx->ptab[s];
         if (N == 128) return launch_mixed<128>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
@@ -573,7 +774,7 @@ std::vector<float> part_table(int N, int P, double side, double corner, double d
     std::vector<float> t(4 * N + R2 * R2);
     for (int i = 0; i < N; ++i) {
         t[i] = (float)El[i];
-        t[N + i] = (float)Fl[i];
+        t[N + i] = (float)(1.0 / Fl[i]);
         t[2 * N + i] = (float)W[i];
         t[3 * N + i] = (float)Vs[i];
     }
@@ -596,54 +797,56 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    const int saved[3] = {ctx->tune[GELL_KNOB_LOD_X_PARTS], ctx->tune[GELL_KNOB_LOD_PARTS],
-                          ctx->tune[GELL_KNOB_LOD_YZ_LDS]};
-    int best[3] = {1, 1, saved[2]};
-    double best_t = 1e30;
-    const bool part_ok = (ctx->N == 128 || ctx->N == 192 || ctx->N == 256 || ctx->N == 512) && !bit_exact;
+    // candidate (x parts, y/z parts, LDS spill, fused)
+    struct Cand {
+        int xp, yzp, sp, fused;
+    };
+    std::vector<Cand> cands;
+    const int N = ctx->N;
+    const bool part_ok = (N == 128 || N == 192 || N == 256 || N == 512) && !bit_exact;
     const int cand[5] = {1, 2, 4, 8, 16};
-    ctx->field[0] = scratch;
-    for (int xi = 0; xi < (part_ok ? 4 : 1) && e == hipSuccess; ++xi)
-        for (int yi = 0; yi < (part_ok ? 5 : 1) && e == hipSuccess; ++yi)
-            for (int sp = 0; sp < 2 && e == hipSuccess; ++sp) {
-                if ((cand[yi] > 1 || ctx->N > 256) && sp == 1) continue;  // the spill exists for serial y/z only
+    for (int xi = 0; xi < (part_ok ? 4 : 1); ++xi)
+        for (int yi = 0; yi < (part_ok ? 5 : 1); ++yi)
+            for (int sp = 0; sp < 2; ++sp) {
+                if ((cand[yi] > 1 || N > 256) && sp == 1) continue;  // the spill exists for serial y/z only
                 const int xp = cand[xi], yzp = cand[yi];
-                if ((xp > 1 && ctx->N % (xp * 32) != 0) || (yzp > 1 && ctx->N % (yzp * 32) != 0)) continue;
-                ctx->tune[GELL_KNOB_LOD_X_PARTS] = xp;
-                ctx->tune[GELL_KNOB_LOD_PARTS] = yzp;
-                ctx->tune[GELL_KNOB_LOD_YZ_LDS] = sp;
-                for (int w = 0; w < 2 && e == hipSuccess; ++w) e = lod_substep_launch(ctx, 0);
-                if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
-                const int reps = 5;
-                for (int r = 0; r < reps && e == hipSuccess; ++r) e = lod_substep_launch(ctx, 0);
-                if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
-                if (e == hipSuccess) e = hipEventSynchronize(b);
-                float ms = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-                const double us = 1e3 * ms / reps;
-                if (e == hipSuccess && us < best_t) {
-                    best_t = us;
-                    best[0] = xp;
-                    best[1] = yzp;
-                    best[2] = sp;
-                }
+                if ((xp > 1 && N % (xp * 32) != 0) || (yzp > 1 && N % (yzp * 32) != 0)) continue;
+                cands.push_back({xp, yzp, sp, 0});
             }
+    if (part_ok && ctx->coup)
+        for (int yi = 1; yi < 5; ++yi)
+            if (N % (cand[yi] * 32) == 0) cands.push_back({N / 64, cand[yi], 0, 1});
+    const int knobs[4] = {GELL_KNOB_LOD_X_PARTS, GELL_KNOB_LOD_PARTS, GELL_KNOB_LOD_YZ_LDS, GELL_KNOB_LOD_FUSED};
+    int saved[4], best[4];
+    for (int q = 0; q < 4; ++q) saved[q] = best[q] = ctx->tune[knobs[q]];
+    double best_t = 1e30;
+    ctx->field[0] = scratch;
+    for (const Cand& c : cands) {
+        if (e != hipSuccess) break;
+        const int val[4] = {c.xp, c.yzp, c.sp, c.fused};
+        for (int q = 0; q < 4; ++q) ctx->tune[knobs[q]] = val[q];
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = lod_substep_launch(ctx, 0);
+        if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+        const int reps = 5;
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = lod_substep_launch(ctx, 0);
+        if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        const double us = 1e3 * ms / reps;
+        if (e == hipSuccess && us < best_t) {
+            best_t = us;
+            for (int q = 0; q < 4; ++q) best[q] = val[q];
+        }
+    }
     ctx->field[0] = real;
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = e2;
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
     (void)hipFree(scratch);
-    if (e == hipSuccess) {
-        ctx->tune[GELL_KNOB_LOD_X_PARTS] = best[0];
-        ctx->tune[GELL_KNOB_LOD_PARTS] = best[1];
-        ctx->tune[GELL_KNOB_LOD_YZ_LDS] = best[2];
-        if (best_us) *best_us = best_t;
-    } else {
-        ctx->tune[GELL_KNOB_LOD_X_PARTS] = saved[0];
-        ctx->tune[GELL_KNOB_LOD_PARTS] = saved[1];
-        ctx->tune[GELL_KNOB_LOD_YZ_LDS] = saved[2];
-    }
+    for (int q = 0; q < 4; ++q) ctx->tune[knobs[q]] = e == hipSuccess ? best[q] : saved[q];
+    if (e == hipSuccess && best_us) *best_us = best_t;
     return e;
 }
 

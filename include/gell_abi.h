@@ -183,13 +183,22 @@ int gell_sync(gell_ctx* ctx);
  *                              Default 4.
  *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep (at most 8).
  *                              Default 4.
- * The first two knobs never change results; gell_autotune sets all four. */
+ *   GELL_KNOB_LOD_FUSED    1 = for N in {128, 256, 512} with LOD_PARTS > 1,
+ *                              the x sweep and the y blocks run in one pass
+ *                              per (z plane, 64 rows), and the y coupling
+ *                              is applied inside the z sweep: 20 bytes of
+ *                              traffic per voxel instead of 28, x and y in
+ *                              N / 64 blocks, z in LOD_PARTS blocks; results
+ *                              equal the unfused kernels with those block
+ *                              counts bit for bit. Default 1.
+ * The first two knobs never change results; gell_autotune sets all of them. */
 enum {
     GELL_KNOB_LOD_YZ_LDS = 0,
     GELL_KNOB_FM_PREFETCH = 1,
     GELL_KNOB_LOD_PARTS = 2,
     GELL_KNOB_LOD_X_PARTS = 3,
-    GELL_N_KNOBS = 4
+    GELL_KNOB_LOD_FUSED = 4,
+    GELL_N_KNOBS = 5
 };
 int gell_tune(gell_ctx* ctx, int knob, int value);
 /* Time every effective LOD variant on a scratch copy of substrate 0's field
@@ -200,7 +209,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value);
 #define GELL_TUNE_BIT_EXACT 1
 int gell_autotune(gell_ctx* ctx, int flags, double* lod_us_out);
 /* Coefficient table of the partitioned line solve for `parts` blocks
- * (GELL_KNOB_LOD_PARTS), host-side: El[N] | Fl[N] | W[N] | V[N] |
+ * (GELL_KNOB_LOD_PARTS), host-side: El[N] | 1/Fl[N] | W[N] | V[N] |
  * Rinv[(2 parts)^2]. For inspection and CPU-side checks; needs no device. */
 int gell_lod_partition_table(const gell_params* params, int32_t substrate, int32_t parts, float* out, int32_t out_len);
 int gell_tune_get(const gell_ctx* ctx, int knob);

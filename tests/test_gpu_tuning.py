@@ -68,6 +68,7 @@ def test_lod_partitioned_within_tolerance(N, xp, yzp, oracle):
     p = (20 + 18 * rng.random(N ** 3)).astype(np.float32)
     rate = (-2 * rng.random(N ** 3) * (rng.random(N ** 3) < 0.3)).astype(np.float32)
     st = GpuState(P, np.zeros(0, rt.CELL_DTYPE))
+    st.ctx.tune("lod_fused", 0)
     st.ctx.tune("lod_x_parts", xp)
     st.ctx.tune("lod_parts", yzp)
     outs = []
@@ -108,3 +109,38 @@ def test_autotune_keeps_state_and_picks_valid_variant(N, oracle):
     want = p.copy()
     oracle.lod_substep(P, 0, want, rate)
     assert rel_l2(got, want) < 1e-6
+
+
+@pytest.mark.parametrize("N,zp", [(128, 2), (128, 4), (256, 4), (256, 2), (256, 8), (512, 8), (512, 16), (512, 4)])
+def test_lod_fused(N, zp, oracle):
+    """The fused substep (x + y blocks in one pass, y coupling applied in the
+    z sweep) gives the bits of the unfused partitioned kernels with x and y in
+    N / 64 blocks and z in zp blocks when zp == N / 64, and stays within the
+    partitioned tolerance of the serial sweep for every zp."""
+    from helpers import rel_l2
+
+    P = params(N)
+    rng = np.random.default_rng(N + zp)
+    p = (20 + 18 * rng.random(N ** 3)).astype(np.float32)
+    rate = (-2 * rng.random(N ** 3) * (rng.random(N ** 3) < 0.3)).astype(np.float32)
+    st = GpuState(P, np.zeros(0, rt.CELL_DTYPE))
+    outs = {}
+    for fused in (1, 0):
+        st.ctx.tune("lod_fused", fused)
+        st.ctx.tune("lod_x_parts", N // 64)
+        st.ctx.tune("lod_parts", zp if fused else N // 64)
+        for rep in range(2 if fused else 1):
+            st.ctx.set_field(0, p)
+            rt.upload_ptr(st.ctx.lib.gell_uptake_field(st.ctx.h, 0), rate)
+            st.ctx.lod_substep(0)
+            outs[(fused, rep)] = st.ctx.field(0)
+    st.close()
+    got = outs[(1, 0)]
+    assert np.array_equal(got.view(np.uint32), outs[(1, 1)].view(np.uint32))
+    if zp == N // 64:
+        assert np.array_equal(got.view(np.uint32), outs[(0, 0)].view(np.uint32))
+    want = p.copy()
+    oracle.lod_substep(P, 0, want, rate)
+    e = rel_l2(got, want)
+    assert e < 1e-6, e
+    assert np.max(np.abs(got - want)) < 1e-4

@@ -14,7 +14,7 @@ def emulate(tab, d, N, P, side, bnd):
     """float32 replay of part_solve for a batch of lines d[lines, N]."""
     f = np.float32
     m = N // P
-    El, Fl, W, V = (tab[q * N:(q + 1) * N] for q in range(4))
+    El, iFl, W, V = (tab[q * N:(q + 1) * N] for q in range(4))
     Rinv = tab[4 * N:].reshape(2 * P, 2 * P)
     d = d.astype(np.float32).copy()
     d[:, 0] = bnd
@@ -25,9 +25,9 @@ def emulate(tab, d, N, P, side, bnd):
         v = d[:, a:a + m].copy()
         for i in range(1, m):
             v[:, i] = v[:, i] - El[a + i] * v[:, i - 1]
-        v[:, m - 1] = v[:, m - 1] / Fl[a + m - 1]
+        v[:, m - 1] = v[:, m - 1] * iFl[a + m - 1]
         for i in range(m - 2, -1, -1):
-            v[:, i] = (v[:, i] - f(side) * v[:, i + 1]) / Fl[a + i]
+            v[:, i] = (v[:, i] - f(side) * v[:, i + 1]) * iFl[a + i]
         y[:, a:a + m] = v
     g = np.zeros((d.shape[0], 2 * P), np.float32)
     for k in range(P):
