@@ -1,0 +1,591 @@
+// nn.hip -- kernels of the LOD surrogate network on gfx950: 3x3x3
+// convolutions as implicit GEMMs on the BF16 matrix cores (FP32
+// accumulation), their weight gradient, and the elementwise pieces (pooling,
+// upsampling, skip adds, ReLU masks, bias gradients, loss, Adam).
+//
+// Tensors are channels-last: element (x, y, z, c) of an X x Y x Z grid with C
+// channels sits at ((z Y + y) X + x) C + c. Activations are BF16 (uint16
+// storage, round-to-nearest-even), parameters FP32 with a BF16 copy in the
+// GEMM layout W[o][k], k = tap C_in + c, tap = (dz 3 + dy) 3 + dx, the input
+// voxel of tap (dx, dy, dz) being the output voxel + (dx - 1, dy - 1, dz - 1),
+// zero outside the grid. Rows are padded to whole 16 x 32 MFMA tiles with
+// zeros.
+//
+// MFMA fragment use (v_mfma_f32_16x16x32_bf16): A = weights (16 output
+// channels x 32 k), B = input (32 k x 16 voxels); lane l supplies row/column
+// l % 16 and the 8 consecutive k of group l / 16 for both operands, and gets
+// D[4 (l / 16) + i][l % 16], i = 0..3: four consecutive output channels of
+// one voxel, stored as one 8-byte write.
+#include <cstdint>
+
+#include "internal.hpp"
+#include "nn.hpp"
+
+namespace gell {
+namespace nn {
+namespace {
+
+typedef __attribute__((ext_vector_type(4))) float f4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// float -> bf16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+constexpr int kThreads = 256;  // 4 waves
+constexpr int kTX = 8, kTY = 8;
+
+// ---------------------------------------------------------------------------
+// Forward convolution. Block = 8 x 8 x TZ output voxels, its input halo
+// (10 x 10 x (TZ + 2) voxels x CIN) staged in LDS as 16-byte chunks; the
+// block's voxels form B / 16 tiles of 16, split over the 4 waves, each wave
+// computing every output-channel tile of its voxel tiles.
+// Epilogue: + bias, ReLU (when relu), BF16 store of channels < cout_store.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int TZ>
+__global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+                                                      const float* __restrict__ bias, uint16_t* __restrict__ y, int X,
+                                                      int Y, int Z, int relu) {
+    constexpr int HX = kTX + 2, HY = kTY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int CH = CIN / 8;
+    constexpr int KP = kpad(CIN);
+    constexpr int KSTEPS = KP / 32;
+    constexpr int NCT = cpad(COUT) / 16;
+    constexpr int NVT = kTX * kTY * TZ / 16;
+    constexpr int VPW = NVT / 4;
+    static_assert(CIN % 8 == 0 && NVT % 4 == 0, "tile shape");
+    __shared__ uint4 halo[HV * CH];
+    const int x0 = blockIdx.x * kTX - 1, y0 = blockIdx.y * kTY - 1, z0 = blockIdx.z * TZ - 1;
+    for (int q = threadIdx.x; q < HV * CH; q += kThreads) {
+        const int ch = q % CH, hv = q / CH;
+        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int gx = x0 + hx, gy = y0 + hy, gz = z0 + hz;
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (gx >= 0 && gx < X && gy >= 0 && gy < Y && gz >= 0 && gz < Z)
+            val = *reinterpret_cast<const uint4*>(x + (((int64_t)gz * Y + gy) * X + gx) * CIN + ch * 8);
+        halo[q] = val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    f4 acc[VPW][NCT];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
+    // in-block position of the voxel this lane feeds as a B column
+    int hbase[VPW];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a) {
+        const int v = (wv + 4 * a) * 16 + r;
+        const int lx = v % kTX, ly = (v / kTX) % kTY, lz = v / (kTX * kTY);
+        hbase[a] = (lz * HY + ly) * HX + lx;
+    }
+    for (int s = 0; s < KSTEPS; ++s) {
+        const int k0 = s * 32 + g * 8;
+        const int tap = k0 / CIN, c0 = k0 % CIN;
+        bf8 af[NCT];
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) af[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + k0);
+        const bool valid = tap < 27;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const int off = (dz * HY + dy) * HX + dx;
+#pragma unroll
+        for (int a = 0; a < VPW; ++a) {
+            uint4 raw = make_uint4(0, 0, 0, 0);
+            if (valid) raw = halo[(hbase[a] + off) * CH + c0 / 8];
+            const bf8 bf = __builtin_bit_cast(bf8, raw);
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf, acc[a][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < VPW; ++a) {
+        const int v = (wv + 4 * a) * 16 + r;
+        const int gx = blockIdx.x * kTX + v % kTX, gy = blockIdx.y * kTY + (v / kTX) % kTY,
+                  gz = blockIdx.z * TZ + v / (kTX * kTY);
+        const int64_t vox = ((int64_t)gz * Y + gy) * X + gx;
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int o = c * 16 + 4 * g;
+            if (o >= COUT) continue;
+            uint16_t h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float t = acc[a][c][i] + bias[o + i];
+                if (relu && t < 0.f) t = 0.f;
+                h[i] = f2bf(t);
+            }
+            uint2 pk;
+            pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+            pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+            *reinterpret_cast<uint2*>(y + vox * COUT + o) = pk;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient dW[o][tap CIN + c] = sum_v dy[v][o] x[v + d(tap)][c] over
+// one block of 8 x 8 x TZ voxels, as MFMAs with the voxels on the K axis:
+// A = dy^T (16 channels x 32 voxels), B = shifted x (32 voxels x 16 k). The
+// block's halo of x and its dy tile sit in LDS; fragments gather 8 voxels per
+// lane. Each block writes its partial sums to part[block][COUTP][KP].
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int TZ>
+__global__ __launch_bounds__(kThreads) void conv3_wgrad(const uint16_t* __restrict__ x,
+                                                        const uint16_t* __restrict__ dy, float* __restrict__ part,
+                                                        int X, int Y, int Z) {
+    constexpr int HX = kTX + 2, HY = kTY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int KP = kpad(CIN);
+    constexpr int COUTP = cpad(COUT);
+    constexpr int NB = kTX * kTY * TZ;   // voxels in the block
+    constexpr int NOT = COUTP / 16;      // output-channel tiles
+    constexpr int NKT = KP / 16;         // k tiles
+    constexpr int NT = NOT * NKT;        // (o, k) tiles
+    constexpr int TPW = (NT + 3) / 4;    // tiles per wave
+    __shared__ uint16_t hx_s[HV * CIN];
+    __shared__ uint16_t dy_s[NB * COUTP];
+    const int x0 = blockIdx.x * kTX - 1, y0 = blockIdx.y * kTY - 1, z0 = blockIdx.z * TZ - 1;
+    for (int q = threadIdx.x; q < HV * (CIN / 8); q += kThreads) {
+        const int ch = q % (CIN / 8), hv = q / (CIN / 8);
+        const int hxx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int gx = x0 + hxx, gy = y0 + hy, gz = z0 + hz;
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (gx >= 0 && gx < X && gy >= 0 && gy < Y && gz >= 0 && gz < Z)
+            val = *reinterpret_cast<const uint4*>(x + (((int64_t)gz * Y + gy) * X + gx) * CIN + ch * 8);
+        *reinterpret_cast<uint4*>(&hx_s[hv * CIN + ch * 8]) = val;
+    }
+    for (int q = threadIdx.x; q < NB * COUTP; q += kThreads) {
+        const int o = q % COUTP, v = q / COUTP;
+        const int gx = blockIdx.x * kTX + v % kTX, gy = blockIdx.y * kTY + (v / kTX) % kTY,
+                  gz = blockIdx.z * TZ + v / (kTX * kTY);
+        dy_s[q] = o < COUT ? dy[(((int64_t)gz * Y + gy) * X + gx) * COUT + o] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    for (int tt = 0; tt < TPW; ++tt) {
+        const int t = wv + 4 * tt;
+        if (t >= NT) break;
+        const int ot = t / NKT, kt = t % NKT;
+        const int o = ot * 16 + r;
+        const int k = kt * 16 + r;
+        const int tap = k / CIN, c = k % CIN;
+        const bool kvalid = tap < 27;
+        const int dz = tap / 9, dyy = (tap / 3) % 3, dx = tap % 3;
+        f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < NB / 32; ++s) {
+            uint16_t av[8], bv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int v = s * 32 + g * 8 + e;
+                av[e] = dy_s[v * COUTP + o];
+                const int lx = v % kTX, ly = (v / kTX) % kTY, lz = v / (kTX * kTY);
+                bv[e] = kvalid ? hx_s[(((lz + dz) * HY + ly + dyy) * HX + lx + dx) * CIN + c] : (uint16_t)0;
+            }
+            bf8 af, bfv;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                af[e] = __builtin_bit_cast(__bf16, av[e]);
+                bfv[e] = __builtin_bit_cast(__bf16, bv[e]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfv, acc, 0, 0, 0);
+        }
+        // acc[i] = dW[ot 16 + 4 g + i][kt 16 + r]
+        const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        float* dst = part + blk * COUTP * KP;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[(ot * 16 + 4 * g + i) * KP + kt * 16 + r] = acc[i];
+    }
+}
+
+// out[i] = sum over blocks of part[b][i] (fixed order: deterministic)
+__global__ void reduce_blocks(const float* __restrict__ part, int64_t nblk, int64_t per, float* __restrict__ out,
+                              int accumulate) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per) return;
+    float s = 0.f;
+    for (int64_t b = 0; b < nblk; ++b) s += part[b * per + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+// W'[c][tap' COUT + o] = W[o][(26 - tap') CIN + c]: the kernel of the input
+// gradient (a convolution of dy with the flipped, transposed weights).
+__global__ void flip_weights(const uint16_t* __restrict__ w, int cin, int cout, uint16_t* __restrict__ wf) {
+    const int kp = kpad(cin), kpf = kpad(cout), coutp_f = cpad(cin);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)coutp_f * kpf) return;
+    const int c = (int)(i / kpf), kf = (int)(i % kpf);
+    const int tapf = kf / cout, o = kf % cout;
+    uint16_t v = 0;
+    if (c < cin && tapf < 27) v = w[o * kp + (26 - tapf) * cin + c];
+    wf[i] = v;
+}
+
+// FP32 master weights (PyTorch-like [o][c][tap] order, see param layout) ->
+// BF16 GEMM layout W[o][tap cin + c], zero padded.
+__global__ void pack_weights(const float* __restrict__ m, int cin, int cout, uint16_t* __restrict__ w) {
+    const int kp = kpad(cin), coutp = cpad(cout);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)coutp * kp) return;
+    const int o = (int)(i / kp), k = (int)(i % kp);
+    const int tap = k / cin, c = k % cin;
+    uint16_t v = 0;
+    if (o < cout && tap < 27) v = f2bf(m[((int64_t)o * cin + c) * 27 + tap]);
+    w[i] = v;
+}
+
+// GEMM-layout gradient -> master layout ([o][c][tap])
+__global__ void unpack_grad(const float* __restrict__ gk, int cin, int cout, float* __restrict__ gm) {
+    const int kp = kpad(cin);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)cout * cin * 27) return;
+    const int tap = (int)(i % 27), c = (int)((i / 27) % cin), o = (int)(i / (27 * cin));
+    gm[i] = gk[o * kp + tap * cin + c];
+}
+
+// dy *= [y > 0] in place (ReLU backward from the layer's own output)
+__global__ void relu_mask(uint16_t* __restrict__ dy, const uint16_t* __restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(bf2f(y[i]) > 0.f)) dy[i] = 0;
+}
+
+// db[o] = sum_v dy[v][o] (one block per channel, fixed order)
+__global__ __launch_bounds__(256) void bias_grad(const uint16_t* __restrict__ dy, int64_t nvox, int cout,
+                                                 float* __restrict__ db) {
+    __shared__ float red[256];
+    const int o = blockIdx.x;
+    float s = 0.f;
+    for (int64_t v = threadIdx.x; v < nvox; v += 256) s += bf2f(dy[v * cout + o]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[o] = red[0];
+}
+
+// 2 x 2 x 2 average pool (out grid X x Y x Z, input 2X x 2Y x 2Z)
+__global__ void pool2(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int X, int Y, int Z, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)X * Y * Z * C) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    float s = 0.f;
+    for (int d = 0; d < 8; ++d) {
+        const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
+        s += bf2f(in[(((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c]);
+    }
+    out[i] = f2bf(s * 0.125f);
+}
+
+// backward of pool2: every fine voxel gets dy / 8
+__global__ void pool2_back(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int X, int Y, int Z, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // fine element
+    const int64_t n = (int64_t)8 * X * Y * Z * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int xi = (int)(v % (2 * X)), yi = (int)((v / (2 * X)) % (2 * Y)), zi = (int)(v / ((int64_t)4 * X * Y));
+    dx[i] = f2bf(bf2f(dy[(((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c]) * 0.125f);
+}
+
+// nearest x2 upsample (out grid 2X x 2Y x 2Z)
+__global__ void up2(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int X, int Y, int Z, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)8 * X * Y * Z * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int xi = (int)(v % (2 * X)), yi = (int)((v / (2 * X)) % (2 * Y)), zi = (int)(v / ((int64_t)4 * X * Y));
+    out[i] = in[(((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c];
+}
+
+// backward of up2: sum of the 8 fine gradients
+__global__ void up2_back(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int X, int Y, int Z, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)X * Y * Z * C) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    float s = 0.f;
+    for (int d = 0; d < 8; ++d) {
+        const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
+        s += bf2f(dy[(((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c]);
+    }
+    dx[i] = f2bf(s);
+}
+
+// out = a + b (bf16); with accumulate, out += a
+__global__ void add_bf(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, uint16_t* __restrict__ out,
+                       int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
+// Input features on the half-resolution grid (X = N / 2): channel 0 = mean of
+// p / bnd - 1 over the 2 x 2 x 2 fine voxels, channel 1 = mean uptake rate
+// times rate_scale, channel 2 = 1 on the coarse voxels of the two outer fine
+// layers (the Dirichlet boundary), channels 3..7 = 0.
+__global__ void features(const float* __restrict__ p, const float* __restrict__ rate, int N, float inv_bnd,
+                         float rate_scale, uint16_t* __restrict__ f) {
+    const int X = N / 2;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)X * X * X) return;
+    const int x = (int)(v % X), y = (int)((v / X) % X), z = (int)(v / ((int64_t)X * X));
+    float sp = 0.f, sr = 0.f;
+    for (int d = 0; d < 8; ++d) {
+        const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
+        const int64_t o = ((int64_t)zi * N + yi) * N + xi;
+        sp += p[o] * inv_bnd - 1.f;
+        sr += rate[o];
+    }
+    const bool edge = x == 0 || y == 0 || z == 0 || x == X - 1 || y == X - 1 || z == X - 1;
+    uint16_t h[8] = {f2bf(sp * 0.125f), f2bf(sr * 0.125f * rate_scale), f2bf(edge ? 1.f : 0.f), 0, 0, 0, 0, 0};
+    uint4 pk;
+    pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+    pk.z = 0;
+    pk.w = 0;
+    *reinterpret_cast<uint4*>(f + v * 8) = pk;
+}
+
+// Full-resolution output: out[fine] = p[fine] + out_scale * head[coarse][sub],
+// sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1); the two outer layers keep the
+// boundary value, as the substep's boundary layers do.
+__global__ void head_out(const float* __restrict__ p, const uint16_t* __restrict__ head, int N, float out_scale,
+                         float bnd, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)N * N * N) return;
+    const int xi = (int)(i % N), yi = (int)((i / N) % N), zi = (int)(i / ((int64_t)N * N));
+    const int X = N / 2;
+    const int sub = (zi & 1) * 4 + (yi & 1) * 2 + (xi & 1);
+    const float d = bf2f(head[(((int64_t)(zi / 2) * X + yi / 2) * X + xi / 2) * 8 + sub]);
+    const bool edge = xi < 2 || yi < 2 || zi < 2 || xi >= N - 2 || yi >= N - 2 || zi >= N - 2;
+    out[i] = edge ? bnd : p[i] + out_scale * d;
+}
+
+// Loss = mean over interior fine voxels of (pred - target)^2 / scale^2, with
+// pred = p + out_scale head; writes dL/dhead (bf16, coarse x 8) and the
+// per-block partial sums of the loss.
+__global__ __launch_bounds__(256) void mse_head(const float* __restrict__ p, const uint16_t* __restrict__ head,
+                                                const float* __restrict__ target, int N, float out_scale,
+                                                float inv_count, uint16_t* __restrict__ dhead,
+                                                float* __restrict__ loss_part) {
+    __shared__ float red[256];
+    const int X = N / 2;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // coarse voxel
+    float l = 0.f;
+    if (c < (int64_t)X * X * X) {
+        const int x = (int)(c % X), y = (int)((c / X) % X), z = (int)(c / ((int64_t)X * X));
+        uint16_t gh[8];
+        for (int sub = 0; sub < 8; ++sub) {
+            const int xi = 2 * x + (sub & 1), yi = 2 * y + ((sub >> 1) & 1), zi = 2 * z + (sub >> 2);
+            const int64_t o = ((int64_t)zi * N + yi) * N + xi;
+            const bool edge = xi < 2 || yi < 2 || zi < 2 || xi >= N - 2 || yi >= N - 2 || zi >= N - 2;
+            const float h = bf2f(head[c * 8 + sub]);
+            // in units of out_scale: e = head - (target - p) / out_scale
+            const float e = edge ? 0.f : h - (target[o] - p[o]) / out_scale;
+            l += e * e;
+            gh[sub] = f2bf(2.f * e * inv_count);
+        }
+        uint4 pk;
+        pk.x = (uint32_t)gh[0] | ((uint32_t)gh[1] << 16);
+        pk.y = (uint32_t)gh[2] | ((uint32_t)gh[3] << 16);
+        pk.z = (uint32_t)gh[4] | ((uint32_t)gh[5] << 16);
+        pk.w = (uint32_t)gh[6] | ((uint32_t)gh[7] << 16);
+        *reinterpret_cast<uint4*>(dhead + c * 8) = pk;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] * inv_count;
+}
+
+// Adam with bias correction (b1, b2, eps), step t >= 1.
+__global__ void adam(float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m1,
+                     float* __restrict__ m2, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gr = grad[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gr;
+    const float b = b2 * m2[i] + (1.f - b2) * gr * gr;
+    m1[i] = a;
+    m2[i] = b;
+    w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+__global__ void f32_to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+__global__ void bf16_to_f32(const uint16_t* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = bf2f(a[i]);
+}
+
+inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// Block depth per input width: halo LDS stays within ~77 KB.
+constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin == 32 ? 4 : (cin == 64 ? 2 : 1)); }
+
+template <int CIN, int COUT>
+hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
+                  hipStream_t st) {
+    constexpr int TZ = tz_for(CIN);
+    if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ>), dim3(X / kTX, Y / kTY, Z / TZ), dim3(kThreads), 0, st, x, w, b, y,
+                       X, Y, Z, relu);
+    return hipGetLastError();
+}
+
+template <int CIN, int COUT>
+hipError_t wgrad_t(const uint16_t* x, const uint16_t* dy, float* part, float* out, int X, int Y, int Z, int accumulate,
+                   hipStream_t st) {
+    constexpr int TZ = tz_for(CIN) < 2 ? 2 : tz_for(CIN);  // 64 voxels min: whole 32-voxel K steps
+    if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
+    const dim3 grid(X / kTX, Y / kTY, Z / TZ);
+    hipLaunchKernelGGL((conv3_wgrad<CIN, COUT, TZ>), grid, dim3(kThreads), 0, st, x, dy, part, X, Y, Z);
+    const int64_t nblk = (int64_t)grid.x * grid.y * grid.z;
+    const int64_t per = (int64_t)cpad(COUT) * kpad(CIN);
+    hipLaunchKernelGGL(reduce_blocks, dim3(blocks_for(per)), dim3(256), 0, st, part, nblk, per, out, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int64_t wgrad_scratch_floats(int cin, int cout, int X, int Y, int Z) {
+    const int tz = tz_for(cin) < 2 ? 2 : tz_for(cin);
+    return (int64_t)(X / kTX) * (Y / kTY) * (Z / tz) * cpad(cout) * kpad(cin);
+}
+
+hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int cin, int cout, int X, int Y,
+                 int Z, int relu, hipStream_t st) {
+#define GELL_NN_CASE(a, c) \
+    if (cin == a && cout == c) return conv_t<a, c>(x, w, b, y, X, Y, Z, relu, st);
+    GELL_NN_CASE(8, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
+    GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64) GELL_NN_CASE(64, 32)
+    GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8)
+#undef GELL_NN_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t conv3_weight_grad(const uint16_t* x, const uint16_t* dy, float* scratch, float* out, int cin, int cout, int X,
+                             int Y, int Z, int accumulate, hipStream_t st) {
+#define GELL_NN_CASE(a, c) \
+    if (cin == a && cout == c) return wgrad_t<a, c>(x, dy, scratch, out, X, Y, Z, accumulate, st);
+    GELL_NN_CASE(8, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
+    GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64) GELL_NN_CASE(64, 32)
+    GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8)
+#undef GELL_NN_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t weights_flip(const uint16_t* w, int cin, int cout, uint16_t* wf, hipStream_t st) {
+    const int64_t n = (int64_t)cpad(cin) * kpad(cout);
+    hipLaunchKernelGGL(flip_weights, dim3(blocks_for(n)), dim3(256), 0, st, w, cin, cout, wf);
+    return hipGetLastError();
+}
+
+hipError_t weights_pack(const float* m, int cin, int cout, uint16_t* w, hipStream_t st) {
+    const int64_t n = (int64_t)cpad(cout) * kpad(cin);
+    hipLaunchKernelGGL(pack_weights, dim3(blocks_for(n)), dim3(256), 0, st, m, cin, cout, w);
+    return hipGetLastError();
+}
+
+hipError_t grad_unpack(const float* gk, int cin, int cout, float* gm, hipStream_t st) {
+    const int64_t n = (int64_t)cout * cin * 27;
+    hipLaunchKernelGGL(unpack_grad, dim3(blocks_for(n)), dim3(256), 0, st, gk, cin, cout, gm);
+    return hipGetLastError();
+}
+
+hipError_t relu_back(uint16_t* dy, const uint16_t* y, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(relu_mask, dim3(blocks_for(n)), dim3(256), 0, st, dy, y, n);
+    return hipGetLastError();
+}
+
+hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, hipStream_t st) {
+    hipLaunchKernelGGL(bias_grad, dim3(cout), dim3(256), 0, st, dy, nvox, cout, db);
+    return hipGetLastError();
+}
+
+hipError_t pool(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int C, hipStream_t st) {
+    hipLaunchKernelGGL(pool2, dim3(blocks_for((int64_t)X * Y * Z * C)), dim3(256), 0, st, in, out, X, Y, Z, C);
+    return hipGetLastError();
+}
+
+hipError_t pool_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st) {
+    hipLaunchKernelGGL(pool2_back, dim3(blocks_for((int64_t)8 * X * Y * Z * C)), dim3(256), 0, st, dy, dx, X, Y, Z, C);
+    return hipGetLastError();
+}
+
+hipError_t upsample(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int C, hipStream_t st) {
+    hipLaunchKernelGGL(up2, dim3(blocks_for((int64_t)8 * X * Y * Z * C)), dim3(256), 0, st, in, out, X, Y, Z, C);
+    return hipGetLastError();
+}
+
+hipError_t upsample_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st) {
+    hipLaunchKernelGGL(up2_back, dim3(blocks_for((int64_t)X * Y * Z * C)), dim3(256), 0, st, dy, dx, X, Y, Z, C);
+    return hipGetLastError();
+}
+
+hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(add_bf, dim3(blocks_for(n)), dim3(256), 0, st, a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t input_features(const float* p, const float* rate, int N, float bnd, float rate_scale, uint16_t* f,
+                          hipStream_t st) {
+    const int64_t n = (int64_t)(N / 2) * (N / 2) * (N / 2);
+    hipLaunchKernelGGL(features, dim3(blocks_for(n)), dim3(256), 0, st, p, rate, N, 1.f / bnd, rate_scale, f);
+    return hipGetLastError();
+}
+
+hipError_t output_field(const float* p, const uint16_t* head, int N, float out_scale, float bnd, float* out,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)N * N * N)), dim3(256), 0, st, p, head, N, out_scale, bnd,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t loss_head(const float* p, const uint16_t* head, const float* target, int N, float out_scale,
+                     uint16_t* dhead, float* loss_part, int64_t* nparts, hipStream_t st) {
+    const int64_t nc = (int64_t)(N / 2) * (N / 2) * (N / 2);
+    const int64_t interior = (int64_t)(N - 4) * (N - 4) * (N - 4);
+    *nparts = blocks_for(nc);
+    hipLaunchKernelGGL(mse_head, dim3((unsigned)*nparts), dim3(256), 0, st, p, head, target, N, out_scale,
+                       1.f / (float)interior, dhead, loss_part);
+    return hipGetLastError();
+}
+
+hipError_t adam_step(float* w, const float* g, float* m1, float* m2, int64_t n, float lr, float b1, float b2, float eps,
+                     int t, hipStream_t st) {
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(adam, dim3(blocks_for(n)), dim3(256), 0, st, w, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t to_bf16(const float* a, uint16_t* b, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(f32_to_bf16, dim3(blocks_for(n)), dim3(256), 0, st, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t to_f32(const uint16_t* a, float* b, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(bf16_to_f32, dim3(blocks_for(n)), dim3(256), 0, st, a, b, n);
+    return hipGetLastError();
+}
+
+}  // namespace nn
+}  // namespace gell

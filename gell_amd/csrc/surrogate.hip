@@ -1,0 +1,537 @@
+// surrogate.hip -- a learned stand-in for K LOD substeps with a fixed uptake
+// map (no counterpart in the reference: its time loop always runs every
+// substep, Main.cu:121-123). The network, on the half-resolution grid
+// (N/2)^3, is a four-level U-Net of 3x3x3 convolutions (BF16 on the matrix
+// cores, FP32 accumulation):
+//
+//   features(p, rate) 8 ch
+//   level 1 (N/2):  L0 8->16, L1 16->16 = e1              pool
+//   level 2 (N/4):  L2 16->32, L3 32->32 = e2             pool
+//   level 3 (N/8):  L4 32->64, L5 64->64 = e3             pool
+//   level 4 (N/16): L6 64->128, L7 128->128 = e4          up
+//   level 3:        L8 128->64 (+ e3), L9 64->64 = d3     up
+//   level 2:        L10 64->32 (+ e2), L11 32->32 = d2    up
+//   level 1:        L12 32->16 (+ e1), L13 16->16 = d1
+//   head:           L14 16->8, no ReLU: the 2x2x2 fine voxels of each coarse
+//                   voxel, added to p (in units of out_scale); the two outer
+//                   layers keep the boundary value.
+//
+// Every layer but the head applies ReLU; the "+ e" skips are added after it.
+// Training (gradients + Adam) runs on the same kernels: input gradients are
+// convolutions with the flipped weights, weight gradients MFMA reductions.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "nn.hpp"
+
+using namespace gell::nn;
+
+namespace {
+
+struct LayerSpec {
+    int cin, cout, level;
+    bool relu;
+};
+
+const LayerSpec kLayers[] = {
+    {8, 16, 1, true},    {16, 16, 1, true},  {16, 32, 2, true},  {32, 32, 2, true},  {32, 64, 3, true},
+    {64, 64, 3, true},   {64, 128, 4, true}, {128, 128, 4, true}, {128, 64, 3, true}, {64, 64, 3, true},
+    {64, 32, 2, true},   {32, 32, 2, true},  {32, 16, 1, true},  {16, 16, 1, true},  {16, 8, 1, false},
+};
+constexpr int kNL = 15;
+
+struct Tensor {
+    uint16_t* d = nullptr;
+    int S = 0, C = 0;  // grid side, channels
+    int64_t n() const { return (int64_t)S * S * S * C; }
+};
+
+}  // namespace
+
+struct gell_surrogate {
+    gell_surrogate_config cfg{};
+    int N = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // parameters: per layer W[cout][cin][27] then b[cout], FP32
+    int64_t w_off[kNL] = {}, b_off[kNL] = {}, nparam = 0;
+    float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    uint16_t* wb[kNL] = {};  // GEMM-layout BF16 weights
+    uint16_t* wf[kNL] = {};  // flipped, for input gradients
+    bool packed = false;     // wb / wf current
+    float* zero_bias = nullptr;
+    // forward tensors
+    Tensor f0, a0, e1, p1, a2, e2, p2, a4, e3, p3, a6, e4, u3, c3, x3, d3, u2, c2, x2, d2, u1, c1, x1, d1, h;
+    // gradient scratch: two per level + head
+    Tensor g1a, g1b, g1c, g2a, g2b, g2c, g3a, g3b, g3c, g4a, g4b, dh;
+    float* wg_scratch = nullptr;
+    int64_t wg_scratch_n = 0;
+    float* wg_out = nullptr;
+    float* loss_part = nullptr;
+    float* h_red = nullptr;  // pinned
+    float* pred = nullptr;   // N^3 scratch for advance
+    float* check = nullptr;  // N^3 scratch for advance
+    float* red_part = nullptr;
+    int adam_t = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+thread_local std::string g_sur_err;
+
+int sfail(gell_surrogate* s, int code, const std::string& m) {
+    if (s) s->err = m;
+    g_sur_err = m;
+    return code;
+}
+
+template <class T>
+hipError_t salloc(gell_surrogate* s, T** p, int64_t count) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), (size_t)(count > 0 ? count : 1) * sizeof(T));
+    if (e == hipSuccess) s->allocs.push_back(*p);
+    return e;
+}
+
+hipError_t tensor(gell_surrogate* s, Tensor& t, int S, int C) {
+    t.S = S;
+    t.C = C;
+    return salloc(s, &t.d, t.n());
+}
+
+#define SCHK(x)                        \
+    do {                               \
+        hipError_t e_ = (x);           \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
+hipError_t pack_all(gell_surrogate* s) {
+    for (int l = 0; l < kNL; ++l) {
+        const LayerSpec& L = kLayers[l];
+        SCHK(weights_pack(s->param + s->w_off[l], L.cin, L.cout, s->wb[l], s->stream));
+        SCHK(weights_flip(s->wb[l], L.cin, L.cout, s->wf[l], s->stream));
+    }
+    s->packed = true;
+    return hipSuccess;
+}
+
+hipError_t conv_layer(gell_surrogate* s, int l, const Tensor& in, Tensor& out) {
+    const LayerSpec& L = kLayers[l];
+    return conv3(in.d, s->wb[l], s->param + s->b_off[l], out.d, L.cin, L.cout, in.S, in.S, in.S, L.relu ? 1 : 0,
+                 s->stream);
+}
+
+hipError_t forward(gell_surrogate* s, const float* p, const float* rate) {
+    if (!s->packed) SCHK(pack_all(s));
+    hipStream_t st = s->stream;
+    SCHK(input_features(p, rate, s->N, s->cfg.boundary, s->cfg.rate_scale, s->f0.d, st));
+    SCHK(conv_layer(s, 0, s->f0, s->a0));
+    SCHK(conv_layer(s, 1, s->a0, s->e1));
+    SCHK(pool(s->e1.d, s->p1.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
+    SCHK(conv_layer(s, 2, s->p1, s->a2));
+    SCHK(conv_layer(s, 3, s->a2, s->e2));
+    SCHK(pool(s->e2.d, s->p2.d, s->p2.S, s->p2.S, s->p2.S, s->p2.C, st));
+    SCHK(conv_layer(s, 4, s->p2, s->a4));
+    SCHK(conv_layer(s, 5, s->a4, s->e3));
+    SCHK(pool(s->e3.d, s->p3.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));
+    SCHK(conv_layer(s, 6, s->p3, s->a6));
+    SCHK(conv_layer(s, 7, s->a6, s->e4));
+    SCHK(upsample(s->e4.d, s->u3.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));
+    SCHK(conv_layer(s, 8, s->u3, s->c3));
+    SCHK(add(s->c3.d, s->e3.d, s->x3.d, s->x3.n(), st));
+    SCHK(conv_layer(s, 9, s->x3, s->d3));
+    SCHK(upsample(s->d3.d, s->u2.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));
+    SCHK(conv_layer(s, 10, s->u2, s->c2));
+    SCHK(add(s->c2.d, s->e2.d, s->x2.d, s->x2.n(), st));
+    SCHK(conv_layer(s, 11, s->x2, s->d2));
+    SCHK(upsample(s->d2.d, s->u1.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));
+    SCHK(conv_layer(s, 12, s->u1, s->c1));
+    SCHK(add(s->c1.d, s->e1.d, s->x1.d, s->x1.n(), st));
+    SCHK(conv_layer(s, 13, s->x1, s->d1));
+    SCHK(conv_layer(s, 14, s->d1, s->h));
+    return hipSuccess;
+}
+
+// Gradients of layer l given dA (gradient at the layer's pre-activation, already
+// masked by ReLU): bias, weights (into the master layout), and, when dx is
+// given, the input gradient.
+hipError_t layer_back(gell_surrogate* s, int l, const Tensor& in, const Tensor& dA, Tensor* dx) {
+    const LayerSpec& L = kLayers[l];
+    hipStream_t st = s->stream;
+    const int S = in.S;
+    SCHK(bias_back(dA.d, (int64_t)S * S * S, L.cout, s->grad + s->b_off[l], st));
+    SCHK(conv3_weight_grad(in.d, dA.d, s->wg_scratch, s->wg_out, L.cin, L.cout, S, S, S, 0, st));
+    SCHK(grad_unpack(s->wg_out, L.cin, L.cout, s->grad + s->w_off[l], st));
+    if (dx) SCHK(conv3(dA.d, s->wf[l], s->zero_bias, dx->d, L.cout, L.cin, S, S, S, 0, st));
+    return hipSuccess;
+}
+
+hipError_t backward(gell_surrogate* s) {
+    hipStream_t st = s->stream;
+    // head (no ReLU): dh holds dL/dh
+    SCHK(layer_back(s, 14, s->d1, s->dh, &s->g1a));                 // g1a = dd1
+    SCHK(relu_back(s->g1a.d, s->d1.d, s->g1a.n(), st));
+    SCHK(layer_back(s, 13, s->x1, s->g1a, &s->g1b));                // g1b = dx1 (= dc1, part of de1)
+    SCHK(hipMemcpyAsync(s->g1c.d, s->g1b.d, s->g1b.n() * 2, hipMemcpyDeviceToDevice, st));  // g1c = de1 so far
+    SCHK(relu_back(s->g1b.d, s->c1.d, s->g1b.n(), st));
+    // The input gradients of L12, L10, L8, L6, L4 and L2 have the shape of
+    // the layer's own input (u1, u2, u3, p3, p2, p1), which the backward pass
+    // no longer needs once that layer's weight gradient is taken: they are
+    // written in place of it.
+    SCHK(layer_back(s, 12, s->u1, s->g1b, &s->u1));                 // u1 <- du1
+    SCHK(upsample_back(s->u1.d, s->g2a.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));  // g2a = dd2
+    SCHK(relu_back(s->g2a.d, s->d2.d, s->g2a.n(), st));
+    SCHK(layer_back(s, 11, s->x2, s->g2a, &s->g2b));                // g2b = dx2
+    SCHK(hipMemcpyAsync(s->g2c.d, s->g2b.d, s->g2b.n() * 2, hipMemcpyDeviceToDevice, st));  // g2c = de2 so far
+    SCHK(relu_back(s->g2b.d, s->c2.d, s->g2b.n(), st));
+    SCHK(layer_back(s, 10, s->u2, s->g2b, &s->u2));                 // u2 <- du2 (64 ch, level 2)
+    SCHK(upsample_back(s->u2.d, s->g3a.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));  // g3a = dd3
+    SCHK(relu_back(s->g3a.d, s->d3.d, s->g3a.n(), st));
+    SCHK(layer_back(s, 9, s->x3, s->g3a, &s->g3b));                 // g3b = dx3
+    SCHK(hipMemcpyAsync(s->g3c.d, s->g3b.d, s->g3b.n() * 2, hipMemcpyDeviceToDevice, st));  // g3c = de3 so far
+    SCHK(relu_back(s->g3b.d, s->c3.d, s->g3b.n(), st));
+    SCHK(layer_back(s, 8, s->u3, s->g3b, &s->u3));                  // u3 <- du3 (128 ch, level 3)
+    SCHK(upsample_back(s->u3.d, s->g4a.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));  // g4a = de4
+    SCHK(relu_back(s->g4a.d, s->e4.d, s->g4a.n(), st));
+    SCHK(layer_back(s, 7, s->a6, s->g4a, &s->g4b));                 // g4b = da6
+    SCHK(relu_back(s->g4b.d, s->a6.d, s->g4b.n(), st));
+    SCHK(layer_back(s, 6, s->p3, s->g4b, &s->p3));                  // p3 <- dp3 (64 ch, level 4)
+    SCHK(pool_back(s->p3.d, s->g3a.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));  // g3a = pool part of de3
+    SCHK(add(s->g3a.d, s->g3c.d, s->g3c.d, s->g3c.n(), st));        // g3c = de3
+    SCHK(relu_back(s->g3c.d, s->e3.d, s->g3c.n(), st));
+    SCHK(layer_back(s, 5, s->a4, s->g3c, &s->g3a));                 // g3a = da4
+    SCHK(relu_back(s->g3a.d, s->a4.d, s->g3a.n(), st));
+    SCHK(layer_back(s, 4, s->p2, s->g3a, &s->p2));                  // p2 <- dp2 (32 ch, level 3)
+    SCHK(pool_back(s->p2.d, s->g2a.d, s->p2.S, s->p2.S, s->p2.S, s->p2.C, st));
+    SCHK(add(s->g2a.d, s->g2c.d, s->g2c.d, s->g2c.n(), st));        // g2c = de2
+    SCHK(relu_back(s->g2c.d, s->e2.d, s->g2c.n(), st));
+    SCHK(layer_back(s, 3, s->a2, s->g2c, &s->g2a));                 // g2a = da2
+    SCHK(relu_back(s->g2a.d, s->a2.d, s->g2a.n(), st));
+    SCHK(layer_back(s, 2, s->p1, s->g2a, &s->p1));                  // p1 <- dp1 (16 ch, level 2)
+    SCHK(pool_back(s->p1.d, s->g1a.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
+    SCHK(add(s->g1a.d, s->g1c.d, s->g1c.d, s->g1c.n(), st));        // g1c = de1
+    SCHK(relu_back(s->g1c.d, s->e1.d, s->g1c.n(), st));
+    SCHK(layer_back(s, 1, s->a0, s->g1c, &s->g1a));                 // g1a = da0
+    SCHK(relu_back(s->g1a.d, s->a0.d, s->g1a.n(), st));
+    SCHK(layer_back(s, 0, s->f0, s->g1a, nullptr));
+    return hipSuccess;
+}
+
+// sum of squares of (a - b) and of a, in fixed-order per-block partials
+__global__ __launch_bounds__(256) void diff_norms(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                                                  float* __restrict__ part) {
+    __shared__ double r0[256], r1[256];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double d = (double)a[i] - (double)b[i];
+        s0 += d * d;
+        s1 += (double)a[i] * (double)a[i];
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (float)r0[0];
+        part[2 * blockIdx.x + 1] = (float)r1[0];
+    }
+}
+
+constexpr int kRedBlocks = 1024;
+
+}  // namespace
+
+extern "C" {
+
+void gell_surrogate_default_config(gell_surrogate_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->N = 256;
+    c->K = 50;
+    c->boundary = 38.f;
+    c->rate_scale = 0.1f;
+    c->out_scale = 1.f;
+    c->seed = 7;
+}
+
+int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_surrogate** out) {
+    if (!out) return sfail(nullptr, GELL_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (!cfg) return sfail(nullptr, GELL_E_ARG, "null config");
+    if (cfg->N < 128 || cfg->N % 128 != 0 || cfg->N > 1024)
+        return sfail(nullptr, GELL_E_UNSUPPORTED, "surrogate grid must be a multiple of 128 (128 .. 1024)");
+    if (cfg->K < 1 || !(cfg->boundary > 0.f) || !(cfg->out_scale > 0.f))
+        return sfail(nullptr, GELL_E_ARG, "bad surrogate config");
+    gell_surrogate* s = new (std::nothrow) gell_surrogate;
+    if (!s) return sfail(nullptr, GELL_E_NOMEM, "surrogate allocation");
+    s->cfg = *cfg;
+    s->N = cfg->N;
+    s->stream = reinterpret_cast<hipStream_t>(stream);
+    int64_t off = 0;
+    for (int l = 0; l < kNL; ++l) {
+        s->w_off[l] = off;
+        off += (int64_t)kLayers[l].cout * kLayers[l].cin * 27;
+        s->b_off[l] = off;
+        off += kLayers[l].cout;
+    }
+    s->nparam = off;
+    const int S1 = s->N / 2, S2 = S1 / 2, S3 = S2 / 2, S4 = S3 / 2;
+    hipError_t e = hipSuccess;
+#define SA(x) \
+    if (e == hipSuccess) e = (x)
+    SA(salloc(s, &s->param, s->nparam));
+    SA(salloc(s, &s->grad, s->nparam));
+    SA(salloc(s, &s->m1, s->nparam));
+    SA(salloc(s, &s->m2, s->nparam));
+    for (int l = 0; l < kNL; ++l) {
+        SA(salloc(s, &s->wb[l], (int64_t)cpad(kLayers[l].cout) * kpad(kLayers[l].cin)));
+        SA(salloc(s, &s->wf[l], (int64_t)cpad(kLayers[l].cin) * kpad(kLayers[l].cout)));
+    }
+    SA(salloc(s, &s->zero_bias, 128));
+    SA(tensor(s, s->f0, S1, 8));
+    SA(tensor(s, s->a0, S1, 16));
+    SA(tensor(s, s->e1, S1, 16));
+    SA(tensor(s, s->p1, S2, 16));
+    SA(tensor(s, s->a2, S2, 32));
+    SA(tensor(s, s->e2, S2, 32));
+    SA(tensor(s, s->p2, S3, 32));
+    SA(tensor(s, s->a4, S3, 64));
+    SA(tensor(s, s->e3, S3, 64));
+    SA(tensor(s, s->p3, S4, 64));
+    SA(tensor(s, s->a6, S4, 128));
+    SA(tensor(s, s->e4, S4, 128));
+    SA(tensor(s, s->u3, S3, 128));
+    SA(tensor(s, s->c3, S3, 64));
+    SA(tensor(s, s->x3, S3, 64));
+    SA(tensor(s, s->d3, S3, 64));
+    SA(tensor(s, s->u2, S2, 64));
+    SA(tensor(s, s->c2, S2, 32));
+    SA(tensor(s, s->x2, S2, 32));
+    SA(tensor(s, s->d2, S2, 32));
+    SA(tensor(s, s->u1, S1, 32));
+    SA(tensor(s, s->c1, S1, 16));
+    SA(tensor(s, s->x1, S1, 16));
+    SA(tensor(s, s->d1, S1, 16));
+    SA(tensor(s, s->h, S1, 8));
+    SA(tensor(s, s->g1a, S1, 16));
+    SA(tensor(s, s->g1b, S1, 16));
+    SA(tensor(s, s->g1c, S1, 16));
+    SA(tensor(s, s->g2a, S2, 32));
+    SA(tensor(s, s->g2b, S2, 32));
+    SA(tensor(s, s->g2c, S2, 32));
+    SA(tensor(s, s->g3a, S3, 64));
+    SA(tensor(s, s->g3b, S3, 64));
+    SA(tensor(s, s->g3c, S3, 64));
+    SA(tensor(s, s->g4a, S4, 128));
+    SA(tensor(s, s->g4b, S4, 128));
+    SA(tensor(s, s->dh, S1, 8));
+    int64_t wgn = 0, wgo = 0;
+    for (int l = 0; l < kNL; ++l) {
+        const int S = s->N >> kLayers[l].level;
+        const int64_t a = wgrad_scratch_floats(kLayers[l].cin, kLayers[l].cout, S, S, S);
+        wgn = a > wgn ? a : wgn;
+        const int64_t b = (int64_t)cpad(kLayers[l].cout) * kpad(kLayers[l].cin);
+        wgo = b > wgo ? b : wgo;
+    }
+    s->wg_scratch_n = wgn;
+    SA(salloc(s, &s->wg_scratch, wgn));
+    SA(salloc(s, &s->wg_out, wgo));
+    SA(salloc(s, &s->loss_part, (int64_t)S1 * S1 * S1 / 256 + 1));
+    SA(salloc(s, &s->pred, (int64_t)s->N * s->N * s->N));
+    SA(salloc(s, &s->check, (int64_t)s->N * s->N * s->N));
+    SA(salloc(s, &s->red_part, 2 * kRedBlocks));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_red), sizeof(float) * (2 * kRedBlocks + 8));
+#undef SA
+    if (e == hipSuccess) {
+        // He-normal initialisation from a seeded host generator; the head
+        // starts at 1/10 of it so the first predictions stay near p.
+        std::vector<float> h(s->nparam, 0.f);
+        std::mt19937_64 rng(cfg->seed);
+        for (int l = 0; l < kNL; ++l) {
+            const double sd = std::sqrt(2.0 / (27.0 * kLayers[l].cin)) * (l == kNL - 1 ? 0.1 : 1.0);
+            std::normal_distribution<double> nd(0.0, sd);
+            for (int64_t i = 0; i < (int64_t)kLayers[l].cout * kLayers[l].cin * 27; ++i)
+                h[s->w_off[l] + i] = (float)nd(rng);
+        }
+        e = hipMemcpy(s->param, h.data(), sizeof(float) * s->nparam, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(s->m1, 0, sizeof(float) * s->nparam);
+        if (e == hipSuccess) e = hipMemset(s->m2, 0, sizeof(float) * s->nparam);
+        if (e == hipSuccess) e = hipMemset(s->grad, 0, sizeof(float) * s->nparam);
+        if (e == hipSuccess) e = hipMemset(s->zero_bias, 0, sizeof(float) * 128);
+    }
+    if (e != hipSuccess) {
+        const std::string m = std::string("surrogate allocation: ") + hipGetErrorString(e);
+        gell_surrogate_destroy(s);
+        return sfail(nullptr, GELL_E_NOMEM, m);
+    }
+    *out = s;
+    return GELL_OK;
+}
+
+void gell_surrogate_destroy(gell_surrogate* s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->h_red) (void)hipHostFree(s->h_red);
+    delete s;
+}
+
+const char* gell_surrogate_last_error(const gell_surrogate* s) { return s ? s->err.c_str() : g_sur_err.c_str(); }
+
+int64_t gell_surrogate_param_count(const gell_surrogate* s) { return s ? s->nparam : -1; }
+
+int gell_surrogate_get_params(gell_surrogate* s, float* host, int64_t n) {
+    if (!s || !host || n != s->nparam) return sfail(s, GELL_E_ARG, "parameter buffer size");
+    hipError_t e = hipMemcpyAsync(host, s->param, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, hipGetErrorString(e));
+}
+
+int gell_surrogate_set_params(gell_surrogate* s, const float* host, int64_t n) {
+    if (!s || !host || n != s->nparam) return sfail(s, GELL_E_ARG, "parameter buffer size");
+    hipError_t e = hipMemcpyAsync(s->param, host, sizeof(float) * n, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    s->packed = false;
+    s->adam_t = 0;
+    if (e == hipSuccess) e = hipMemsetAsync(s->m1, 0, sizeof(float) * n, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->m2, 0, sizeof(float) * n, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, hipGetErrorString(e));
+}
+
+int gell_surrogate_predict(gell_surrogate* s, const float* field, const float* rate, float* out) {
+    if (!s || !field || !rate || !out) return sfail(s, GELL_E_ARG, "null argument");
+    hipError_t e = forward(s, field, rate);
+    if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, out, s->stream);
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("predict: ") + hipGetErrorString(e));
+}
+
+int gell_surrogate_gradients(gell_surrogate* s, const float* field, const float* rate, const float* target,
+                             float* loss, float* grad_host) {
+    if (!s || !field || !rate || !target) return sfail(s, GELL_E_ARG, "null argument");
+    hipError_t e = forward(s, field, rate);
+    int64_t np = 0;
+    if (e == hipSuccess)
+        e = loss_head(field, s->h.d, target, s->N, s->cfg.out_scale, s->dh.d, s->loss_part, &np, s->stream);
+    if (e == hipSuccess) e = backward(s);
+    if (e == hipSuccess && loss) {
+        std::vector<float> lp(np);
+        e = hipMemcpyAsync(lp.data(), s->loss_part, sizeof(float) * np, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        double acc = 0.0;
+        for (float v : lp) acc += v;
+        *loss = (float)acc;
+    }
+    if (e == hipSuccess && grad_host) {
+        e = hipMemcpyAsync(grad_host, s->grad, sizeof(float) * s->nparam, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("gradients: ") + hipGetErrorString(e));
+}
+
+int gell_surrogate_adam(gell_surrogate* s, float lr) {
+    if (!s) return sfail(nullptr, GELL_E_ARG, "null surrogate");
+    ++s->adam_t;
+    hipError_t e = adam_step(s->param, s->grad, s->m1, s->m2, s->nparam, lr, 0.9f, 0.999f, 1e-8f, s->adam_t, s->stream);
+    s->packed = false;
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("adam: ") + hipGetErrorString(e));
+}
+
+int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float tol, int32_t* used, float* residual) {
+    if (!s || !ctx || !used) return sfail(s, GELL_E_ARG, "null argument");
+    if (ctx->N != s->N) return sfail(s, GELL_E_ARG, "grid size differs from the surrogate's");
+    if (sub < 0 || sub >= ctx->P.n_substrates) return sfail(s, GELL_E_ARG, "substrate out of range");
+    if (ctx->stream != s->stream) return sfail(s, GELL_E_ARG, "surrogate and context must share a stream");
+    hipStream_t st = s->stream;
+    const int64_t V = ctx->V;
+    float* const field = ctx->field[sub];
+    hipError_t e = forward(s, field, ctx->rate[sub]);
+    if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, s->pred, st);
+    // one more substep from the prediction: its relative change is the residual
+    if (e == hipSuccess) e = hipMemcpyAsync(s->check, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) {
+        ctx->field[sub] = s->check;
+        e = gell::lod_substep_launch(ctx, sub);
+        ctx->field[sub] = field;
+    }
+    if (e == hipSuccess) hipLaunchKernelGGL(diff_norms, dim3(kRedBlocks), dim3(256), 0, st, s->check, s->pred, V, s->red_part);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(s->h_red, s->red_part, sizeof(float) * 2 * kRedBlocks, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < kRedBlocks; ++b) {
+        num += s->h_red[2 * b];
+        den += s->h_red[2 * b + 1];
+    }
+    const float res = den > 0.0 ? (float)std::sqrt(num / den) : 0.f;
+    if (residual) *residual = res;
+    *used = res <= tol ? 1 : 0;
+    if (*used) {
+        e = hipMemcpyAsync(field, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
+    } else {
+        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub);
+    }
+    return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
+}
+
+// Single 3x3x3 layer on an S^3 grid, for checking the kernels one at a time.
+// x, y BF16 device tensors; w_master FP32 [cout][cin][27] and b FP32 host or
+// device per `params_on_host`. Allocates and frees its own packing buffers;
+// synchronises.
+int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
+                  int32_t S, int32_t relu, int32_t input_grad) {
+    if (!x || !w_master || !b || !y || S < 8 || S % 8) return sfail(nullptr, GELL_E_ARG, "bad conv arguments");
+    const int64_t nw = (int64_t)cout * cin * 27;
+    float *dm = nullptr, *db = nullptr;
+    uint16_t *wb = nullptr, *wf = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dm), sizeof(float) * nw);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&db), sizeof(float) * 128);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&wb), 2 * (size_t)cpad(cout) * kpad(cin));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&wf), 2 * (size_t)cpad(cin) * kpad(cout));
+    if (e == hipSuccess) e = hipMemcpy(dm, w_master, sizeof(float) * nw, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(db, 0, sizeof(float) * 128);
+    if (e == hipSuccess && !input_grad) e = hipMemcpy(db, b, sizeof(float) * cout, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = weights_pack(dm, cin, cout, wb, nullptr);
+    if (e == hipSuccess && input_grad) e = weights_flip(wb, cin, cout, wf, nullptr);
+    // input_grad: y (cin channels) = input gradient of the layer for x = dy (cout channels)
+    if (e == hipSuccess)
+        e = input_grad ? conv3(x, wf, db, y, cout, cin, S, S, S, 0, nullptr)
+                       : conv3(x, wb, db, y, cin, cout, S, S, S, relu, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    for (void* p : {(void*)dm, (void*)db, (void*)wb, (void*)wf})
+        if (p) (void)hipFree(p);
+    return e == hipSuccess ? GELL_OK : sfail(nullptr, GELL_E_HIP, std::string("conv3: ") + hipGetErrorString(e));
+}
+
+// Weight gradient of one layer: dw_master (host, [cout][cin][27]) =
+// sum_v dy[v][o] x[v + tap][c]. Allocates; synchronises.
+int gell_nn_conv3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw_master, int32_t cin, int32_t cout,
+                        int32_t S) {
+    if (!x || !dy || !dw_master || S < 8 || S % 8) return sfail(nullptr, GELL_E_ARG, "bad wgrad arguments");
+    const int64_t ns = wgrad_scratch_floats(cin, cout, S, S, S);
+    const int64_t nk = (int64_t)cpad(cout) * kpad(cin), nw = (int64_t)cout * cin * 27;
+    float *scr = nullptr, *gk = nullptr, *gm = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&scr), sizeof(float) * ns);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&gk), sizeof(float) * nk);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&gm), sizeof(float) * nw);
+    if (e == hipSuccess) e = conv3_weight_grad(x, dy, scr, gk, cin, cout, S, S, S, 0, nullptr);
+    if (e == hipSuccess) e = grad_unpack(gk, cin, cout, gm, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(dw_master, gm, sizeof(float) * nw, hipMemcpyDeviceToHost);
+    for (void* p : {(void*)scr, (void*)gk, (void*)gm})
+        if (p) (void)hipFree(p);
+    return e == hipSuccess ? GELL_OK : sfail(nullptr, GELL_E_HIP, std::string("wgrad: ") + hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -69,6 +69,18 @@ class Params(C.Structure):
     ]
 
 
+class SurrogateConfig(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32),
+        ("K", C.c_int32),
+        ("boundary", C.c_float),
+        ("rate_scale", C.c_float),
+        ("out_scale", C.c_float),
+        ("reserved_", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
 class GellError(RuntimeError):
     pass
 
@@ -134,6 +146,19 @@ def load_library() -> C.CDLL:
         "gell_ensemble_bio_step": (C.c_int, [vp, P(vp), P(i32), u64]),
         "gell_ensemble_sync": (C.c_int, [vp]),
         "gell_ensemble_stats": (C.c_int, [vp, P(i64), P(i64)]),
+        "gell_surrogate_default_config": (None, [P(SurrogateConfig)]),
+        "gell_surrogate_create": (C.c_int, [P(SurrogateConfig), vp, P(vp)]),
+        "gell_surrogate_destroy": (None, [vp]),
+        "gell_surrogate_last_error": (C.c_char_p, [vp]),
+        "gell_surrogate_param_count": (i64, [vp]),
+        "gell_surrogate_get_params": (C.c_int, [vp, vp, i64]),
+        "gell_surrogate_set_params": (C.c_int, [vp, vp, i64]),
+        "gell_surrogate_predict": (C.c_int, [vp, vp, vp, vp]),
+        "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, P(C.c_float), vp]),
+        "gell_surrogate_adam": (C.c_int, [vp, C.c_float]),
+        "gell_surrogate_advance": (C.c_int, [vp, vp, i32, C.c_float, P(i32), P(C.c_float)]),
+        "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
+        "gell_nn_conv3_wgrad": (C.c_int, [vp, vp, vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -440,6 +465,87 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class Surrogate:
+    """The LOD surrogate network (gell_surrogate_* of the C ABI)."""
+
+    def __init__(self, N: int, K: int = 50, stream: int | None = None, **cfg):
+        self.lib = load_library()
+        c = SurrogateConfig()
+        self.lib.gell_surrogate_default_config(C.byref(c))
+        c.N, c.K = N, K
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        h = C.c_void_p()
+        st = self.lib.gell_surrogate_create(C.byref(c), stream, C.byref(h))
+        if st != 0:
+            raise GellError(f"gell_surrogate_create: {self.lib.gell_surrogate_last_error(None).decode()} ({st})")
+        self.h = h.value
+        self.N = N
+        self.nparam = self.lib.gell_surrogate_param_count(self.h)
+
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            raise GellError(f"{what}: {self.lib.gell_surrogate_last_error(self.h).decode()} ({st})")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, np.float32)
+        self._check(self.lib.gell_surrogate_get_params(self.h, out.ctypes.data, self.nparam), "get_params")
+        return out
+
+    def set_params(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, np.float32)
+        self._check(self.lib.gell_surrogate_set_params(self.h, a.ctypes.data, a.size), "set_params")
+
+    def predict(self, field: int, rate: int, out: int) -> None:
+        self._check(self.lib.gell_surrogate_predict(self.h, _ptr(field), _ptr(rate), _ptr(out)), "predict")
+
+    def gradients(self, field, rate, target, want_grad: bool = False):
+        loss = C.c_float()
+        g = np.zeros(self.nparam, np.float32) if want_grad else None
+        self._check(self.lib.gell_surrogate_gradients(self.h, _ptr(field), _ptr(rate), _ptr(target), C.byref(loss),
+                                                      g.ctypes.data if want_grad else None), "gradients")
+        return (loss.value, g) if want_grad else loss.value
+
+    def adam(self, lr: float) -> None:
+        self._check(self.lib.gell_surrogate_adam(self.h, lr), "adam")
+
+    def advance(self, ctx: "Context", s: int, tol: float = 1e-3) -> tuple[bool, float]:
+        used, res = C.c_int32(), C.c_float()
+        self._check(self.lib.gell_surrogate_advance(self.h, ctx.h, s, tol, C.byref(used), C.byref(res)), "advance")
+        return bool(used.value), res.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.gell_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def nn_conv3(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, cin: int, cout: int, S: int,
+             relu: bool = False, input_grad: bool = False) -> None:
+    """One network layer alone (x, y device BF16 tensors, w [cout][cin][27], b[cout] host FP32)."""
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    st = load_library().gell_nn_conv3(_ptr(x), w.ctypes.data, b.ctypes.data, _ptr(y), cin, cout, S, int(relu),
+                                      int(input_grad))
+    if st != 0:
+        raise GellError(f"gell_nn_conv3: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
+
+
+def nn_conv3_wgrad(x: DeviceBuffer, dy: DeviceBuffer, cin: int, cout: int, S: int) -> np.ndarray:
+    out = np.zeros((cout, cin, 27), np.float32)
+    st = load_library().gell_nn_conv3_wgrad(_ptr(x), _ptr(dy), out.ctypes.data, cin, cout, S)
+    if st != 0:
+        raise GellError(f"gell_nn_conv3_wgrad: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
+    return out
 
 
 def _ptr(x) -> int:

@@ -280,6 +280,61 @@ int gell_ensemble_sync(gell_ensemble* ens);
 /* Number of graph captures and graph launches so far. */
 int gell_ensemble_stats(const gell_ensemble* ens, int64_t* captures, int64_t* launches);
 
+/* ---------------------------------------------------------------------------
+ * LOD surrogate (no counterpart in the reference, whose loop runs every
+ * substep, Main.cu:121-123): a four-level 3-D U-Net (16..128 channels, BF16
+ * matrix-core convolutions with FP32 accumulation) on the (N/2)^3 grid that
+ * maps (field, uptake map) to the field after K substeps with that map held
+ * fixed. Forward, gradients and Adam run on the device; the in-loop call
+ * checks each prediction with one real substep and falls back to K substeps
+ * when the relative change exceeds the tolerance.
+ * ------------------------------------------------------------------------- */
+typedef struct gell_surrogate gell_surrogate;
+typedef struct {
+    int32_t N;        /* fine grid (multiple of 128) */
+    int32_t K;        /* substeps one prediction stands for */
+    float boundary;   /* boundary value (input scaling; outer layers) */
+    float rate_scale; /* input scaling of the uptake map */
+    float out_scale;  /* field units of one head unit */
+    int32_t reserved_;
+    uint64_t seed;    /* weight initialisation */
+} gell_surrogate_config;
+void gell_surrogate_default_config(gell_surrogate_config* cfg);
+/* `stream` (a hipStream_t) is the stream every surrogate call runs on. */
+int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_surrogate** out);
+void gell_surrogate_destroy(gell_surrogate* sur);
+const char* gell_surrogate_last_error(const gell_surrogate* sur);
+/* FP32 parameters, per layer W[cout][cin][27] (tap = (dz 3 + dy) 3 + dx) then b[cout]. */
+int64_t gell_surrogate_param_count(const gell_surrogate* sur);
+int gell_surrogate_get_params(gell_surrogate* sur, float* host, int64_t n);
+int gell_surrogate_set_params(gell_surrogate* sur, const float* host, int64_t n);
+/* out = predicted field after K substeps (device pointers to N^3 floats). */
+int gell_surrogate_predict(gell_surrogate* sur, const float* field, const float* rate, float* out);
+/* Loss (mean squared interior error in out_scale units) and parameter
+ * gradients for one sample; gradients copied to grad_host when not NULL
+ * (param_count floats). Synchronises. */
+int gell_surrogate_gradients(gell_surrogate* sur, const float* field, const float* rate, const float* target,
+                             float* loss, float* grad_host);
+/* One Adam step (0.9, 0.999, 1e-8) on the last gradients. */
+int gell_surrogate_adam(gell_surrogate* sur, float lr);
+/* Advance substrate `s` of ctx by K substeps: the surrogate's prediction is
+ * kept when one real substep changes it by at most `tol` (relative L2),
+ * otherwise K LOD substeps run instead; *used tells which, *residual the
+ * measured change. ctx must have grid N and run on the surrogate's stream.
+ * Synchronises. */
+int gell_surrogate_advance(gell_surrogate* sur, gell_ctx* ctx, int32_t s, float tol, int32_t* used, float* residual);
+/* One of the network's 3x3x3 layers alone on an S^3 grid (S a multiple of
+ * 8), for checking the kernels: x, y device BF16 tensors (channels-last);
+ * w_master [cout][cin][27] and b[cout] on the host. With input_grad the call
+ * computes the layer's input gradient instead: x = dy (cout channels), y =
+ * dx (cin channels). Supported (cin, cout): the network's layers and their
+ * reverses. Allocates; synchronises. */
+int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
+                  int32_t S, int32_t relu, int32_t input_grad);
+/* The layer's weight gradient into host dw_master ([cout][cin][27]). */
+int gell_nn_conv3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw_master, int32_t cin, int32_t cout,
+                        int32_t S);
+
 #ifdef __cplusplus
 }
 #endif

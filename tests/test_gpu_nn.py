@@ -1,0 +1,142 @@
+"""The surrogate network's device kernels against the CPU restatement
+(oracle/nn_oracle.py) with BF16 rounding emulated where the device rounds:
+each 3x3x3 layer shape (forward, input gradient, weight gradient), then the
+whole network's prediction, loss and parameter gradients, then a few Adam
+steps. Tolerances: one BF16 rounding step (2^-8 relative) on BF16 outputs,
+1e-5 on FP32-accumulated weight gradients of exact BF16 products, and
+5e-2 relative L2 on whole-network gradients (15 layers of BF16 activations
+and gradients)."""
+import numpy as np
+import pytest
+
+from gell_amd import runtime as rt
+import nn_oracle as nno
+from helpers import log, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 64), (64, 32),
+          (32, 16), (16, 8)]
+
+
+def to_bf16_bits(a):
+    a32 = np.ascontiguousarray(a, np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bits(h):
+    return (h.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def dev(h):
+    b = rt.DeviceBuffer(h.nbytes)
+    b.upload(h)
+    return b
+
+
+def close_bf16(got, want):
+    scale = max(np.abs(want).max(), 1e-6)
+    err = np.abs(got - want)
+    return np.all(err <= 2.0 ** -7 * np.abs(want) + 2.0 ** -9 * 1e-2 * scale), float(err.max() / scale)
+
+
+@pytest.mark.parametrize("cin,cout", SHAPES)
+def test_conv_layer_forward_and_gradients(cin, cout):
+    S = 16 if cin <= 64 else 8
+    rng = np.random.default_rng(cin * 1000 + cout)
+    xb = to_bf16_bits(rng.normal(size=(S, S, S, cin)))
+    x = from_bits(xb)
+    W = nno.bf16(rng.normal(0, np.sqrt(2 / (27 * cin)), size=(cout, cin, 27)))
+    b = rng.normal(0, 0.1, size=cout).astype(np.float32).astype(np.float64)
+    xd = dev(xb)
+    yd = rt.DeviceBuffer(S ** 3 * cout * 2)
+    rt.nn_conv3(xd, W, b, yd, cin, cout, S, relu=True)
+    got = from_bits(yd.download(np.uint16, S ** 3 * cout)).reshape(S, S, S, cout)
+    want = nno.bf16(nno.conv3(x, W, b, True))
+    ok, e = close_bf16(got, want)
+    assert ok, f"forward: max err {e:.3e}"
+    # input gradient: dx = conv of dy with the flipped weights
+    dyb = to_bf16_bits(rng.normal(size=(S, S, S, cout)))
+    dy = from_bits(dyb)
+    dyd = dev(dyb)
+    dxd = rt.DeviceBuffer(S ** 3 * cin * 2)
+    rt.nn_conv3(dyd, W, np.zeros(cout), dxd, cin, cout, S, input_grad=True)
+    gdx = from_bits(dxd.download(np.uint16, S ** 3 * cin)).reshape(S, S, S, cin)
+    wdx, wdW, _ = nno.conv3_back(x, W, dy)
+    ok, e = close_bf16(gdx, nno.bf16(wdx))
+    assert ok, f"input gradient: max err {e:.3e}"
+    # weight gradient (exact BF16 products, FP32 sums)
+    gdW = rt.nn_conv3_wgrad(xd, dyd, cin, cout, S)
+    assert rel_l2(gdW.astype(np.float64), wdW) < 1e-5
+    for buf in (xd, yd, dyd, dxd):
+        buf.free()
+
+
+CFG = dict(boundary=38.0, rate_scale=0.1, out_scale=1.0)
+
+
+@pytest.fixture(scope="module")
+def net_problem():
+    N = 128
+    rng = np.random.default_rng(11)
+    p = (38.0 - 6.0 * rng.random((N, N, N))).astype(np.float32)
+    rate = (-25.0 * (rng.random((N, N, N)) < 0.1)).astype(np.float32)
+    target = (p - 2.0 * rng.random((N, N, N))).astype(np.float32)
+    sur = rt.Surrogate(N)
+    params = sur.params().astype(np.float64)
+    # non-zero biases so the bias paths carry signal
+    w_off, b_off, _ = nno.param_layout()
+    for l, (cin, cout, _, _) in enumerate(nno.LAYERS):
+        params[b_off[l]:b_off[l] + cout] = 0.05 * rng.normal(size=cout)
+    params = params.astype(np.float32)
+    sur.set_params(params)
+    bufs = [dev(a) for a in (p, rate, target)]
+    yield N, p, rate, target, params.astype(np.float64), sur, bufs
+    for b in bufs:
+        b.free()
+    sur.close()
+
+
+def test_network_prediction_matches_oracle(net_problem):
+    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    out = rt.DeviceBuffer(N ** 3 * 4)
+    sur.predict(pd, rd, out)
+    got = out.download(np.float32, N ** 3).reshape(N, N, N).astype(np.float64)
+    want = nno.forward(params, p.astype(np.float64), rate.astype(np.float64), CFG, rnd=nno.bf16)
+    e = rel_l2(got - p, want - p)
+    log(f"[nn] prediction rel L2 of the update vs oracle: {e:.3e}")
+    assert e < 1e-2
+    out.free()
+
+
+def test_network_gradients_match_oracle(net_problem):
+    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    loss, g = sur.gradients(pd, rd, td, want_grad=True)
+    wl, wg = nno.loss_and_grad(params, p.astype(np.float64), rate.astype(np.float64), target.astype(np.float64), CFG,
+                               rnd=nno.bf16)
+    log(f"[nn] loss device {loss:.6f} oracle {wl:.6f}")
+    assert abs(loss - wl) <= 1e-2 * wl
+    w_off, b_off, _ = nno.param_layout()
+    worst = 0.0
+    for l, (cin, cout, _, _) in enumerate(nno.LAYERS):
+        sw = slice(w_off[l], w_off[l] + cout * cin * 27)
+        sb = slice(b_off[l], b_off[l] + cout)
+        ew, eb = rel_l2(g[sw], wg[sw]), rel_l2(g[sb], wg[sb])
+        worst = max(worst, ew, eb)
+        log(f"[nn] layer {l} ({cin}->{cout}): weight grad rel L2 {ew:.2e}, bias grad {eb:.2e}")
+    assert worst < 5e-2
+
+
+def test_adam_steps_reduce_the_loss(net_problem):
+    """A uniform shift of the interior is learnable through the head bias
+    alone; a few Adam steps must cut the loss in half."""
+    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    shifted = dev((p - 0.1).astype(np.float32))
+    losses = []
+    for _ in range(12):
+        losses.append(sur.gradients(pd, rd, shifted))
+        sur.adam(5e-3)
+    log(f"[nn] losses {losses[0]:.5f} -> {losses[-1]:.5f}")
+    assert losses[-1] < 0.5 * losses[0]
+    shifted.free()
