@@ -341,12 +341,143 @@ def ensemble_main(args) -> int:
     return 0
 
 
+SURROGATE_WEIGHTS = ROOT / "gell_amd" / "weights" / "lod_surrogate_k50.npy"
+
+
+def surrogate_main(args) -> int:
+    """Config N: the 256^3 / 1M-cell spheroid (oxygen), one biology step with
+    plain LOD substeps against the same step with the surrogate standing in
+    for each run of K = 50 substeps (uptake map of the mechanics step it
+    starts in; a prediction whose one-substep relative change exceeds the
+    tolerance falls back to the 50 substeps). Both runs start from the same
+    state after the warm-up steps."""
+    from gell_amd import runtime as rt
+
+    cfg = dict(CONFIGS["L"], n_sub=1)
+    P, cells = make_workload(cfg, args.seed)
+    n0 = cfg["cells"]
+    ctx = rt.Context(P)
+    ctx.autotune()
+    buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
+    buf.upload(cells)
+    K = 50
+    sur = rt.Surrogate(cfg["N"], K)
+    sur.set_params(np.load(args.weights, allow_pickle=False))
+    mech = rt.float_loop_count(P.dt_bio, P.dt_mech)
+    diff = rt.float_loop_count(P.dt_mech, P.dt_diff)
+    n = n0
+    for w in range(args.warmup):
+        n = ctx.birth(buf, n, w)
+        for _ in range(mech):
+            ctx.fm_update(buf, n)
+            ctx.prelod(0, buf, n)
+            for _ in range(diff):
+                ctx.lod_substep(0)
+        n = ctx.death(buf, n)
+    ctx.sync()
+    snap_cells = buf.download(rt.CELL_DTYPE, P.capacity)
+    snap_field = ctx.field(0)
+    snap_n = n
+
+    def bio_step(use_sur: bool, step: int):
+        stats = dict(calls=0, used=0, residual=[], lod_substeps=0)
+        m = ctx.birth(buf, snap_n, step)
+        born = m - snap_n
+        owed = 0
+        for _ in range(mech):
+            ctx.fm_update(buf, m)
+            ctx.prelod(0, buf, m)
+            if use_sur:
+                owed += diff
+                while owed >= K:
+                    used, res = sur.advance(ctx, 0, args.sur_tol)
+                    stats["calls"] += 1
+                    stats["used"] += int(used)
+                    stats["residual"].append(res)
+                    stats["lod_substeps"] += 0 if used else K
+                    owed -= K
+            else:
+                for _ in range(diff):
+                    ctx.lod_substep(0)
+                stats["lod_substeps"] += diff
+        for _ in range(owed):
+            ctx.lod_substep(0)
+        stats["lod_substeps"] += owed
+        m2 = ctx.death(buf, m)
+        ctx.sync()
+        stats["births"], stats["deaths"], stats["n_end"] = born, m - m2, m2
+        return stats
+
+    out = {}
+    for mode in ("plain", "surrogate"):
+        buf.upload(snap_cells)
+        ctx.set_field(0, snap_field)
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = bio_step(mode == "surrogate", args.warmup)
+        st["seconds"] = time.perf_counter() - t0
+        st["field"] = ctx.field(0)
+        out[mode] = st
+        log(f"[bench N] {mode}: {st['seconds'] * 1e3:.1f} ms, calls {st['calls']}, used {st['used']}")
+    # the surrogate alone against the 50 substeps it replaces
+    ctx.set_field(0, snap_field)
+    pred = rt.DeviceBuffer(cfg["N"] ** 3 * 4)
+    rate_ptr = ctx.lib.gell_uptake_field(ctx.h, 0)
+    for _ in range(2):
+        sur.predict(ctx.field_ptr(0), rate_ptr, pred)
+    ctx.sync()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sur.predict(ctx.field_ptr(0), rate_ptr, pred)
+    ctx.sync()
+    t_pred = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ctx.lod_substep(0)
+    ctx.sync()
+    t_lod = time.perf_counter() - t0
+    pl, su = out["plain"], out["surrogate"]
+    res = su["residual"]
+    line = {
+        "metric": "surrogate vs LOD at 1e-3 L2 (K = 50 substeps, 256^3, 1M cells)",
+        "value": t_lod / t_pred,
+        "unit": "x (50 LOD substeps / one surrogate prediction)",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "bf16 (network) / f32 (field)",
+        "data": "synthetic (seeded spheroid)",
+        "config": {"workload": "N: 256^3 at 10 um, 1000000 cells in a 1200 um ball, oxygen; surrogate every 50 "
+                               "substeps", "tolerance": args.sur_tol, "weights": str(args.weights)},
+        "predict_ms": t_pred * 1e3,
+        "lod50_ms": t_lod * 1e3,
+        "bio_step_ms": {"plain": pl["seconds"] * 1e3, "surrogate": su["seconds"] * 1e3},
+        "surrogate_calls": su["calls"],
+        "surrogate_accepted": su["used"],
+        "residual_median": float(np.median(res)) if res else None,
+        "residual_max": float(np.max(res)) if res else None,
+        "field_rel_l2_vs_plain": float(np.linalg.norm(su["field"].astype(np.float64) - pl["field"])
+                                       / np.linalg.norm(pl["field"].astype(np.float64))),
+        "births": {"plain": pl["births"], "surrogate": su["births"]},
+        "deaths": {"plain": pl["deaths"], "surrogate": su["deaths"]},
+        "lod_substeps": {"plain": pl["lod_substeps"], "surrogate": su["lod_substeps"]},
+    }
+    print(json.dumps(line), flush=True)
+    pred.free()
+    buf.free()
+    sur.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=2, help="timed biology steps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed biology steps")
-    ap.add_argument("--config", choices=sorted(CONFIGS) + ["B"], default="L")
+    ap.add_argument("--config", choices=sorted(CONFIGS) + ["B", "N"], default="L")
+    ap.add_argument("--weights", default=str(SURROGATE_WEIGHTS), help="surrogate parameters (config N)")
+    ap.add_argument("--sur-tol", type=float, default=1e-3, help="surrogate acceptance tolerance (config N)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
     ap.add_argument("--only", choices=["lod", "fm", "copy"], default=None,
@@ -363,6 +494,8 @@ def main() -> int:
 
     if args.config == "B":
         return ensemble_main(args)
+    if args.config == "N":
+        return surrogate_main(args)
     cfg = CONFIGS[args.config]
     log(f"[bench] building workload {args.config}")
     P, cells = make_workload(cfg, args.seed)

@@ -242,12 +242,13 @@ __global__ void pack_weights(const float* __restrict__ m, int cin, int cout, uin
 }
 
 // GEMM-layout gradient -> master layout ([o][c][tap])
-__global__ void unpack_grad(const float* __restrict__ gk, int cin, int cout, float* __restrict__ gm) {
+__global__ void unpack_grad(const float* __restrict__ gk, int cin, int cout, float* __restrict__ gm, int accumulate) {
     const int kp = kpad(cin);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)cout * cin * 27) return;
     const int tap = (int)(i % 27), c = (int)((i / 27) % cin), o = (int)(i / (27 * cin));
-    gm[i] = gk[o * kp + tap * cin + c];
+    const float v = gk[o * kp + tap * cin + c];
+    gm[i] = accumulate ? gm[i] + v : v;
 }
 
 // dy *= [y > 0] in place (ReLU backward from the layer's own output)
@@ -258,7 +259,7 @@ __global__ void relu_mask(uint16_t* __restrict__ dy, const uint16_t* __restrict_
 
 // db[o] = sum_v dy[v][o] (one block per channel, fixed order)
 __global__ __launch_bounds__(256) void bias_grad(const uint16_t* __restrict__ dy, int64_t nvox, int cout,
-                                                 float* __restrict__ db) {
+                                                 float* __restrict__ db, int accumulate) {
     __shared__ float red[256];
     const int o = blockIdx.x;
     float s = 0.f;
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void bias_grad(const uint16_t* __restrict__ dy
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[o] = red[0];
+    if (threadIdx.x == 0) db[o] = accumulate ? db[o] + red[0] : red[0];
 }
 
 // 2 x 2 x 2 average pool (out grid X x Y x Z, input 2X x 2Y x 2Z)
@@ -331,36 +332,39 @@ __global__ void add_bf(const uint16_t* __restrict__ a, const uint16_t* __restric
     if (i < n) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
 
-// Input features on the half-resolution grid (X = N / 2): channel 0 = mean of
-// p / bnd - 1 over the 2 x 2 x 2 fine voxels, channel 1 = mean uptake rate
-// times rate_scale, channel 2 = 1 on the coarse voxels of the two outer fine
-// layers (the Dirichlet boundary), channels 3..7 = 0.
+// Input features on the half-resolution grid (X = N / 2), 24 channels per
+// coarse voxel: q = p / bnd - 1 at its 8 fine voxels split into a BF16 high
+// part (channels 0..7) and the BF16 rounding of the remainder (8..15), so the
+// pair carries ~16 significant bits; then the uptake rate times rate_scale at
+// the same 8 (16..23). Fine voxel d = (z & 1) 4 + (y & 1) 2 + (x & 1).
 __global__ void features(const float* __restrict__ p, const float* __restrict__ rate, int N, float inv_bnd,
                          float rate_scale, uint16_t* __restrict__ f) {
     const int X = N / 2;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (int64_t)X * X * X) return;
     const int x = (int)(v % X), y = (int)((v / X) % X), z = (int)(v / ((int64_t)X * X));
-    float sp = 0.f, sr = 0.f;
+    uint16_t h[24];
     for (int d = 0; d < 8; ++d) {
         const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
         const int64_t o = ((int64_t)zi * N + yi) * N + xi;
-        sp += p[o] * inv_bnd - 1.f;
-        sr += rate[o];
+        const float q = p[o] * inv_bnd - 1.f;
+        h[d] = f2bf(q);
+        h[8 + d] = f2bf(q - bf2f(h[d]));
+        h[16 + d] = f2bf(rate[o] * rate_scale);
     }
-    const bool edge = x == 0 || y == 0 || z == 0 || x == X - 1 || y == X - 1 || z == X - 1;
-    uint16_t h[8] = {f2bf(sp * 0.125f), f2bf(sr * 0.125f * rate_scale), f2bf(edge ? 1.f : 0.f), 0, 0, 0, 0, 0};
-    uint4 pk;
-    pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-    pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-    pk.z = 0;
-    pk.w = 0;
-    *reinterpret_cast<uint4*>(f + v * 8) = pk;
+    uint4 pk[3];
+    for (int q = 0; q < 3; ++q) {
+        pk[q].x = (uint32_t)h[8 * q + 0] | ((uint32_t)h[8 * q + 1] << 16);
+        pk[q].y = (uint32_t)h[8 * q + 2] | ((uint32_t)h[8 * q + 3] << 16);
+        pk[q].z = (uint32_t)h[8 * q + 4] | ((uint32_t)h[8 * q + 5] << 16);
+        pk[q].w = (uint32_t)h[8 * q + 6] | ((uint32_t)h[8 * q + 7] << 16);
+    }
+    for (int q = 0; q < 3; ++q) reinterpret_cast<uint4*>(f + v * 24)[q] = pk[q];
 }
 
 // Full-resolution output: out[fine] = p[fine] + out_scale * head[coarse][sub],
-// sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1); the two outer layers keep the
-// boundary value, as the substep's boundary layers do.
+// sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1); the two outer layers, which every
+// substep resets, keep p.
 __global__ void head_out(const float* __restrict__ p, const uint16_t* __restrict__ head, int N, float out_scale,
                          float bnd, float* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,7 +374,7 @@ __global__ void head_out(const float* __restrict__ p, const uint16_t* __restrict
     const int sub = (zi & 1) * 4 + (yi & 1) * 2 + (xi & 1);
     const float d = bf2f(head[(((int64_t)(zi / 2) * X + yi / 2) * X + xi / 2) * 8 + sub]);
     const bool edge = xi < 2 || yi < 2 || zi < 2 || xi >= N - 2 || yi >= N - 2 || zi >= N - 2;
-    out[i] = edge ? bnd : p[i] + out_scale * d;
+    out[i] = edge ? p[i] : p[i] + out_scale * d;
 }
 
 // Loss = mean over interior fine voxels of (pred - target)^2 / scale^2, with
@@ -439,7 +443,7 @@ __global__ void bf16_to_f32(const uint16_t* __restrict__ a, float* __restrict__ 
 inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // Block depth per input width: halo LDS stays within ~77 KB.
-constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin == 32 ? 4 : (cin == 64 ? 2 : 1)); }
+constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin <= 32 ? 4 : (cin == 64 ? 2 : 1)); }
 
 template <int CIN, int COUT>
 hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
@@ -475,9 +479,9 @@ hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t*
                  int Z, int relu, hipStream_t st) {
 #define GELL_NN_CASE(a, c) \
     if (cin == a && cout == c) return conv_t<a, c>(x, w, b, y, X, Y, Z, relu, st);
-    GELL_NN_CASE(8, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
-    GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64) GELL_NN_CASE(64, 32)
-    GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8)
+    GELL_NN_CASE(24, 16) GELL_NN_CASE(16, 24) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32)
+    GELL_NN_CASE(32, 64) GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64)
+    GELL_NN_CASE(64, 32) GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8) GELL_NN_CASE(8, 16)
 #undef GELL_NN_CASE
     return hipErrorInvalidValue;
 }
@@ -486,7 +490,7 @@ hipError_t conv3_weight_grad(const uint16_t* x, const uint16_t* dy, float* scrat
                              int Y, int Z, int accumulate, hipStream_t st) {
 #define GELL_NN_CASE(a, c) \
     if (cin == a && cout == c) return wgrad_t<a, c>(x, dy, scratch, out, X, Y, Z, accumulate, st);
-    GELL_NN_CASE(8, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
+    GELL_NN_CASE(24, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
     GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64) GELL_NN_CASE(64, 32)
     GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8)
 #undef GELL_NN_CASE
@@ -505,9 +509,9 @@ hipError_t weights_pack(const float* m, int cin, int cout, uint16_t* w, hipStrea
     return hipGetLastError();
 }
 
-hipError_t grad_unpack(const float* gk, int cin, int cout, float* gm, hipStream_t st) {
+hipError_t grad_unpack(const float* gk, int cin, int cout, float* gm, int accumulate, hipStream_t st) {
     const int64_t n = (int64_t)cout * cin * 27;
-    hipLaunchKernelGGL(unpack_grad, dim3(blocks_for(n)), dim3(256), 0, st, gk, cin, cout, gm);
+    hipLaunchKernelGGL(unpack_grad, dim3(blocks_for(n)), dim3(256), 0, st, gk, cin, cout, gm, accumulate);
     return hipGetLastError();
 }
 
@@ -516,8 +520,8 @@ hipError_t relu_back(uint16_t* dy, const uint16_t* y, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, hipStream_t st) {
-    hipLaunchKernelGGL(bias_grad, dim3(cout), dim3(256), 0, st, dy, nvox, cout, db);
+hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, int accumulate, hipStream_t st) {
+    hipLaunchKernelGGL(bias_grad, dim3(cout), dim3(256), 0, st, dy, nvox, cout, db, accumulate);
     return hipGetLastError();
 }
 

@@ -27,10 +27,10 @@ int64_t wgrad_scratch_floats(int cin, int cout, int X, int Y, int Z);
 hipError_t weights_flip(const uint16_t* w, int cin, int cout, uint16_t* wf, hipStream_t st);
 // master [cout][cin][27] FP32 -> GEMM BF16, and GEMM FP32 gradient -> master layout
 hipError_t weights_pack(const float* m, int cin, int cout, uint16_t* w, hipStream_t st);
-hipError_t grad_unpack(const float* gk, int cin, int cout, float* gm, hipStream_t st);
+hipError_t grad_unpack(const float* gk, int cin, int cout, float* gm, int accumulate, hipStream_t st);
 
 hipError_t relu_back(uint16_t* dy, const uint16_t* y, int64_t n, hipStream_t st);
-hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, hipStream_t st);
+hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, int accumulate, hipStream_t st);
 // grids given at the coarse side (X x Y x Z; the fine side is twice that)
 hipError_t pool(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int C, hipStream_t st);
 hipError_t pool_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st);
@@ -38,7 +38,7 @@ hipError_t upsample(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int 
 hipError_t upsample_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st);
 hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, hipStream_t st);
 
-// N^3 field + uptake map -> 8-channel features on the (N/2)^3 grid
+// N^3 field + uptake map -> 24-channel features on the (N/2)^3 grid
 hipError_t input_features(const float* p, const float* rate, int N, float bnd, float rate_scale, uint16_t* f,
                           hipStream_t st);
 // (N/2)^3 x 8 head -> N^3 field: p + out_scale head (sub-voxel order z, y, x)

@@ -4,8 +4,9 @@
 // (N/2)^3, is a four-level U-Net of 3x3x3 convolutions (BF16 on the matrix
 // cores, FP32 accumulation):
 //
-//   features(p, rate) 8 ch
-//   level 1 (N/2):  L0 8->16, L1 16->16 = e1              pool
+//   features(p, rate) 24 ch: p (as BF16 high + low parts) and rate at the 8
+//                   fine voxels of each coarse one
+//   level 1 (N/2):  L0 24->16, L1 16->16 = e1             pool
 //   level 2 (N/4):  L2 16->32, L3 32->32 = e2             pool
 //   level 3 (N/8):  L4 32->64, L5 64->64 = e3             pool
 //   level 4 (N/16): L6 64->128, L7 128->128 = e4          up
@@ -38,7 +39,7 @@ struct LayerSpec {
 };
 
 const LayerSpec kLayers[] = {
-    {8, 16, 1, true},    {16, 16, 1, true},  {16, 32, 2, true},  {32, 32, 2, true},  {32, 64, 3, true},
+    {24, 16, 1, true},   {16, 16, 1, true},  {16, 32, 2, true},  {32, 32, 2, true},  {32, 64, 3, true},
     {64, 64, 3, true},   {64, 128, 4, true}, {128, 128, 4, true}, {128, 64, 3, true}, {64, 64, 3, true},
     {64, 32, 2, true},   {32, 32, 2, true},  {32, 16, 1, true},  {16, 16, 1, true},  {16, 8, 1, false},
 };
@@ -77,6 +78,7 @@ struct gell_surrogate {
     float* check = nullptr;  // N^3 scratch for advance
     float* red_part = nullptr;
     int adam_t = 0;
+    int accumulate = 0;  // add this call's gradients to the stored ones
     std::vector<void*> allocs;
 };
 
@@ -164,9 +166,9 @@ hipError_t layer_back(gell_surrogate* s, int l, const Tensor& in, const Tensor& 
     const LayerSpec& L = kLayers[l];
     hipStream_t st = s->stream;
     const int S = in.S;
-    SCHK(bias_back(dA.d, (int64_t)S * S * S, L.cout, s->grad + s->b_off[l], st));
+    SCHK(bias_back(dA.d, (int64_t)S * S * S, L.cout, s->grad + s->b_off[l], s->accumulate, st));
     SCHK(conv3_weight_grad(in.d, dA.d, s->wg_scratch, s->wg_out, L.cin, L.cout, S, S, S, 0, st));
-    SCHK(grad_unpack(s->wg_out, L.cin, L.cout, s->grad + s->w_off[l], st));
+    SCHK(grad_unpack(s->wg_out, L.cin, L.cout, s->grad + s->w_off[l], s->accumulate, st));
     if (dx) SCHK(conv3(dA.d, s->wf[l], s->zero_bias, dx->d, L.cout, L.cin, S, S, S, 0, st));
     return hipSuccess;
 }
@@ -299,7 +301,7 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
         SA(salloc(s, &s->wf[l], (int64_t)cpad(kLayers[l].cin) * kpad(kLayers[l].cout)));
     }
     SA(salloc(s, &s->zero_bias, 128));
-    SA(tensor(s, s->f0, S1, 8));
+    SA(tensor(s, s->f0, S1, 24));
     SA(tensor(s, s->a0, S1, 16));
     SA(tensor(s, s->e1, S1, 16));
     SA(tensor(s, s->p1, S2, 16));
@@ -355,11 +357,11 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
 #undef SA
     if (e == hipSuccess) {
         // He-normal initialisation from a seeded host generator; the head
-        // starts at 1/10 of it so the first predictions stay near p.
+        // starts at 1/100 of it so the first predictions stay near p.
         std::vector<float> h(s->nparam, 0.f);
         std::mt19937_64 rng(cfg->seed);
         for (int l = 0; l < kNL; ++l) {
-            const double sd = std::sqrt(2.0 / (27.0 * kLayers[l].cin)) * (l == kNL - 1 ? 0.1 : 1.0);
+            const double sd = std::sqrt(2.0 / (27.0 * kLayers[l].cin)) * (l == kNL - 1 ? 0.01 : 1.0);
             std::normal_distribution<double> nd(0.0, sd);
             for (int64_t i = 0; i < (int64_t)kLayers[l].cout * kLayers[l].cin * 27; ++i)
                 h[s->w_off[l] + i] = (float)nd(rng);
@@ -418,8 +420,9 @@ int gell_surrogate_predict(gell_surrogate* s, const float* field, const float* r
 }
 
 int gell_surrogate_gradients(gell_surrogate* s, const float* field, const float* rate, const float* target,
-                             float* loss, float* grad_host) {
+                             int32_t accumulate, float* loss, float* grad_host) {
     if (!s || !field || !rate || !target) return sfail(s, GELL_E_ARG, "null argument");
+    s->accumulate = accumulate ? 1 : 0;
     hipError_t e = forward(s, field, rate);
     int64_t np = 0;
     if (e == hipSuccess)
@@ -527,7 +530,7 @@ int gell_nn_conv3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw_master,
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&gk), sizeof(float) * nk);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&gm), sizeof(float) * nw);
     if (e == hipSuccess) e = conv3_weight_grad(x, dy, scr, gk, cin, cout, S, S, S, 0, nullptr);
-    if (e == hipSuccess) e = grad_unpack(gk, cin, cout, gm, nullptr);
+    if (e == hipSuccess) e = grad_unpack(gk, cin, cout, gm, 0, nullptr);
     if (e == hipSuccess) e = hipMemcpy(dw_master, gm, sizeof(float) * nw, hipMemcpyDeviceToHost);
     for (void* p : {(void*)scr, (void*)gk, (void*)gm})
         if (p) (void)hipFree(p);

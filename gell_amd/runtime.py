@@ -154,7 +154,7 @@ def load_library() -> C.CDLL:
         "gell_surrogate_get_params": (C.c_int, [vp, vp, i64]),
         "gell_surrogate_set_params": (C.c_int, [vp, vp, i64]),
         "gell_surrogate_predict": (C.c_int, [vp, vp, vp, vp]),
-        "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, P(C.c_float), vp]),
+        "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, i32, P(C.c_float), vp]),
         "gell_surrogate_adam": (C.c_int, [vp, C.c_float]),
         "gell_surrogate_advance": (C.c_int, [vp, vp, i32, C.c_float, P(i32), P(C.c_float)]),
         "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
@@ -502,11 +502,12 @@ class Surrogate:
     def predict(self, field: int, rate: int, out: int) -> None:
         self._check(self.lib.gell_surrogate_predict(self.h, _ptr(field), _ptr(rate), _ptr(out)), "predict")
 
-    def gradients(self, field, rate, target, want_grad: bool = False):
+    def gradients(self, field, rate, target, want_grad: bool = False, accumulate: bool = False):
         loss = C.c_float()
         g = np.zeros(self.nparam, np.float32) if want_grad else None
-        self._check(self.lib.gell_surrogate_gradients(self.h, _ptr(field), _ptr(rate), _ptr(target), C.byref(loss),
-                                                      g.ctypes.data if want_grad else None), "gradients")
+        self._check(self.lib.gell_surrogate_gradients(self.h, _ptr(field), _ptr(rate), _ptr(target), int(accumulate),
+                                                      C.byref(loss), g.ctypes.data if want_grad else None),
+                    "gradients")
         return (loss.value, g) if want_grad else loss.value
 
     def adam(self, lr: float) -> None:

@@ -311,10 +311,11 @@ int gell_surrogate_set_params(gell_surrogate* sur, const float* host, int64_t n)
 /* out = predicted field after K substeps (device pointers to N^3 floats). */
 int gell_surrogate_predict(gell_surrogate* sur, const float* field, const float* rate, float* out);
 /* Loss (mean squared interior error in out_scale units) and parameter
- * gradients for one sample; gradients copied to grad_host when not NULL
- * (param_count floats). Synchronises. */
+ * gradients for one sample, replacing the stored gradients or, with
+ * accumulate, added to them (mini-batches); the stored gradients are copied
+ * to grad_host when not NULL (param_count floats). Synchronises. */
 int gell_surrogate_gradients(gell_surrogate* sur, const float* field, const float* rate, const float* target,
-                             float* loss, float* grad_host);
+                             int32_t accumulate, float* loss, float* grad_host);
 /* One Adam step (0.9, 0.999, 1e-8) on the last gradients. */
 int gell_surrogate_adam(gell_surrogate* sur, float lr);
 /* Advance substrate `s` of ctx by K substeps: the surrogate's prediction is

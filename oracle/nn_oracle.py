@@ -19,7 +19,7 @@ from __future__ import annotations
 import numpy as np
 
 LAYERS = [  # (cin, cout, level, relu)
-    (8, 16, 1, True), (16, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True),
+    (24, 16, 1, True), (16, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True),
     (64, 64, 3, True), (64, 128, 4, True), (128, 128, 4, True), (128, 64, 3, True), (64, 64, 3, True),
     (64, 32, 2, True), (32, 32, 2, True), (32, 16, 1, True), (16, 16, 1, True), (16, 8, 1, False),
 ]
@@ -114,18 +114,14 @@ def up_back(dy, rnd=ident):
 
 
 def features(p, rate, bnd, rate_scale, rnd=ident):
-    """8-channel input on the half grid (fine arrays indexed [z, y, x])."""
-    N = p.shape[0]
-    X = N // 2
-    f = np.zeros((X, X, X, 8))
-    f[..., 0] = (p / bnd - 1.0).reshape(X, 2, X, 2, X, 2).mean(axis=(1, 3, 5))
-    f[..., 1] = rate.reshape(X, 2, X, 2, X, 2).mean(axis=(1, 3, 5)) * rate_scale
-    edge = np.zeros((X, X, X), bool)
-    edge[[0, -1], :, :] = True
-    edge[:, [0, -1], :] = True
-    edge[:, :, [0, -1]] = True
-    f[..., 2] = edge
-    return rnd(f)
+    """24-channel input on the half grid (fine arrays indexed [z, y, x]):
+    q = p / bnd - 1 at the 8 fine voxels of each coarse voxel as a rounded
+    high part and the rounded remainder, then rate times rate_scale at the
+    same 8 (fine voxel (z & 1) 4 + (y & 1) 2 + (x & 1)). Without rounding the
+    remainder channels are zero."""
+    q = fine_to_head(p / bnd - 1.0)
+    hi = rnd(q)
+    return np.concatenate([hi, rnd(q - hi), rnd(fine_to_head(rate * rate_scale))], axis=-1)
 
 
 def head_to_fine(h):
@@ -177,7 +173,7 @@ def forward(params, p, rate, cfg, rnd=ident, keep=False):
     t["d1"] = c(13, t["x1"])
     t["h"] = c(14, t["d1"])
     out = p + cfg["out_scale"] * head_to_fine(t["h"])
-    out = np.where(interior(p.shape[0]), out, cfg["boundary"])
+    out = np.where(interior(p.shape[0]), out, p)
     return (out, t, wr) if keep else out
 
 
@@ -246,11 +242,11 @@ def loss_and_grad(params, p, rate, target, cfg, rnd=ident):
 
 
 def init_params(seed=7):
-    """He-normal weights (head scaled by 0.1), zero biases."""
+    """He-normal weights (head scaled by 0.01), zero biases."""
     rng = np.random.default_rng(seed)
     w_off, b_off, total = param_layout()
     out = np.zeros(total)
     for l, (cin, cout, _, _) in enumerate(LAYERS):
-        sd = np.sqrt(2.0 / (27 * cin)) * (0.1 if l == len(LAYERS) - 1 else 1.0)
+        sd = np.sqrt(2.0 / (27 * cin)) * (0.01 if l == len(LAYERS) - 1 else 1.0)
         out[w_off[l]:w_off[l] + cout * cin * 27] = rng.normal(0.0, sd, cout * cin * 27)
     return out
