@@ -57,6 +57,9 @@ CONFIGS = {
 }
 
 
+# Trained parameters of the LOD surrogate (K = 50 substeps, oxygen).
+SURROGATE_WEIGHTS = ROOT / "gell_amd" / "weights" / "lod_surrogate_k50.npy"
+
 # Batched ensemble: 64 replicates, n log-uniform in [1k, 200k], each in the
 # smallest grid of {64, 96, 128, 192}^3 (10 um voxels) that holds its ball
 # (density of the 1M-cell / 1.2 mm ball) with a 50 um margin; seed + r.
@@ -186,9 +189,10 @@ def cpu_leg(cfg, P, cells, budget_s):
 
 def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | None = None) -> dict:
     """One kernel group alone, for profiler runs: `lod` (LOD substeps of
-    substrate 0 after one mechanics step + uptake map), `fm` (mechanics steps)
-    or `copy` (the streaming copy kernel over 256 MiB, the byte-counter
-    calibration)."""
+    substrate 0 after one mechanics step + uptake map), `fm` (mechanics steps),
+    `surrogate` (predictions of the committed LOD surrogate on that field and
+    uptake map) or `copy` (the streaming copy kernel over 256 MiB, the
+    byte-counter calibration)."""
     from gell_amd import runtime as rt
 
     n = cfg["cells"]
@@ -202,7 +206,21 @@ def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | No
         ctx.prelod(s, buf, n)
     ctx.sync()
     out = {"what": what, "steps": steps}
-    if what == "copy":
+    if what == "surrogate":
+        sur = rt.Surrogate(cfg["N"], 50)
+        sur.set_params(np.load(SURROGATE_WEIGHTS, allow_pickle=False))
+        pred = rt.DeviceBuffer(cfg["N"] ** 3 * 4)
+        rate_ptr = ctx.lib.gell_uptake_field(ctx.h, 0)
+        for k in range(warmup + steps):
+            if k == warmup:
+                ctx.sync()
+                t0 = time.perf_counter()
+            sur.predict(ctx.field_ptr(0), rate_ptr, pred)
+        ctx.sync()
+        out["predict_ms"] = (time.perf_counter() - t0) / max(steps, 1) * 1e3
+        pred.free()
+        sur.close()
+    elif what == "copy":
         nbytes = 256 << 20
         a, b = rt.DeviceBuffer(nbytes), rt.DeviceBuffer(nbytes)
         for _ in range(warmup + steps):
@@ -341,9 +359,6 @@ def ensemble_main(args) -> int:
     return 0
 
 
-SURROGATE_WEIGHTS = ROOT / "gell_amd" / "weights" / "lod_surrogate_k50.npy"
-
-
 def surrogate_main(args) -> int:
     """Config N: the 256^3 / 1M-cell spheroid (oxygen), one biology step with
     plain LOD substeps against the same step with the surrogate standing in
@@ -480,7 +495,7 @@ def main() -> int:
     ap.add_argument("--sur-tol", type=float, default=1e-3, help="surrogate acceptance tolerance (config N)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
-    ap.add_argument("--only", choices=["lod", "fm", "copy"], default=None,
+    ap.add_argument("--only", choices=["lod", "fm", "copy", "surrogate"], default=None,
                     help="run one kernel group alone (profiler runs); prints its timings")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="kernel-variant knob for --only runs (e.g. lod_parts=8)")

@@ -87,12 +87,19 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
         const int lx = v % kTX, ly = (v / kTX) % kTY, lz = v / (kTX * kTY);
         hbase[a] = (lz * HY + ly) * HX + lx;
     }
+    // weight fragments of step s + 1 are loaded while step s computes
+    bf8 af[NCT], an[NCT];
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) an[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + g * 8);
     for (int s = 0; s < KSTEPS; ++s) {
         const int k0 = s * 32 + g * 8;
         const int tap = k0 / CIN, c0 = k0 % CIN;
-        bf8 af[NCT];
 #pragma unroll
-        for (int c = 0; c < NCT; ++c) af[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + k0);
+        for (int c = 0; c < NCT; ++c) af[c] = an[c];
+        if (s + 1 < KSTEPS) {
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) an[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + k0 + 32);
+        }
         const bool valid = tap < 27;
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
         const int off = (dz * HY + dy) * HX + dx;
