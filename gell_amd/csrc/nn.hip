@@ -42,13 +42,18 @@ constexpr int kThreads = 256;  // 4 waves
 constexpr int kTX = 8, kTY = 8;
 
 // ---------------------------------------------------------------------------
-// Forward convolution. Block = 8 x 8 x TZ output voxels, its input halo
-// (10 x 10 x (TZ + 2) voxels x CIN) staged in LDS as 16-byte chunks; the
-// block's voxels form B / 16 tiles of 16, split over the 4 waves, each wave
-// computing every output-channel tile of its voxel tiles.
-// Epilogue: + bias, ReLU (when relu), BF16 store of channels < cout_store.
+// Forward convolution. Block = 8 x 8 x TZ output voxels and CG of the layer's
+// 16-channel output tiles (blockIdx.z also walks the NCT / CG tile groups);
+// its input halo (10 x 10 x (TZ + 2) voxels x CIN) is staged in LDS as
+// 16-byte chunks. The block's voxels form B / 16 tiles of 16, split over the
+// 4 waves; each wave computes its voxel tiles for the block's CG output tiles,
+// so one LDS fragment feeds CG matrix-core operations. Grids of 256 blocks or
+// more take every output tile in one block (CG = NCT); smaller ones (the
+// 16^3 level) one tile per block (CG = 1), for more blocks and fewer weight
+// loads per wave.
+// Epilogue: + bias, ReLU (when relu), BF16 store.
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int TZ>
+template <int CIN, int COUT, int TZ, int CG>
 __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                       const float* __restrict__ bias, uint16_t* __restrict__ y, int X,
                                                       int Y, int Z, int relu) {
@@ -59,9 +64,11 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
     constexpr int NCT = cpad(COUT) / 16;
     constexpr int NVT = kTX * kTY * TZ / 16;
     constexpr int VPW = NVT / 4;
-    static_assert(CIN % 8 == 0 && NVT % 4 == 0, "tile shape");
+    static_assert(CIN % 8 == 0 && NVT % 4 == 0 && NCT % CG == 0, "tile shape");
     __shared__ uint4 halo[HV * CH];
-    const int x0 = blockIdx.x * kTX - 1, y0 = blockIdx.y * kTY - 1, z0 = blockIdx.z * TZ - 1;
+    const int zb = Z / TZ;
+    const int bz = blockIdx.z % zb, ct0 = (blockIdx.z / zb) * CG;
+    const int x0 = blockIdx.x * kTX - 1, y0 = blockIdx.y * kTY - 1, z0 = bz * TZ - 1;
     for (int q = threadIdx.x; q < HV * CH; q += kThreads) {
         const int ch = q % CH, hv = q / CH;
         const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
@@ -74,11 +81,11 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    f4 acc[VPW][NCT];
+    f4 acc[VPW][CG];
 #pragma unroll
     for (int a = 0; a < VPW; ++a)
 #pragma unroll
-        for (int c = 0; c < NCT; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < CG; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
     // in-block position of the voxel this lane feeds as a B column
     int hbase[VPW];
 #pragma unroll
@@ -87,18 +94,19 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
         const int lx = v % kTX, ly = (v / kTX) % kTY, lz = v / (kTX * kTY);
         hbase[a] = (lz * HY + ly) * HX + lx;
     }
+    const uint16_t* wr = w + ((ct0 * 16) + r) * KP + g * 8;
     // weight fragments of step s + 1 are loaded while step s computes
-    bf8 af[NCT], an[NCT];
+    bf8 af[CG], an[CG];
 #pragma unroll
-    for (int c = 0; c < NCT; ++c) an[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + g * 8);
+    for (int c = 0; c < CG; ++c) an[c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP);
     for (int s = 0; s < KSTEPS; ++s) {
         const int k0 = s * 32 + g * 8;
         const int tap = k0 / CIN, c0 = k0 % CIN;
 #pragma unroll
-        for (int c = 0; c < NCT; ++c) af[c] = an[c];
+        for (int c = 0; c < CG; ++c) af[c] = an[c];
         if (s + 1 < KSTEPS) {
 #pragma unroll
-            for (int c = 0; c < NCT; ++c) an[c] = *reinterpret_cast<const bf8*>(w + (c * 16 + r) * KP + k0 + 32);
+            for (int c = 0; c < CG; ++c) an[c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP + (s + 1) * 32);
         }
         const bool valid = tap < 27;
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
@@ -109,18 +117,18 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
             if (valid) raw = halo[(hbase[a] + off) * CH + c0 / 8];
             const bf8 bf = __builtin_bit_cast(bf8, raw);
 #pragma unroll
-            for (int c = 0; c < NCT; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf, acc[a][c], 0, 0, 0);
+            for (int c = 0; c < CG; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf, acc[a][c], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int a = 0; a < VPW; ++a) {
         const int v = (wv + 4 * a) * 16 + r;
         const int gx = blockIdx.x * kTX + v % kTX, gy = blockIdx.y * kTY + (v / kTX) % kTY,
-                  gz = blockIdx.z * TZ + v / (kTX * kTY);
+                  gz = bz * TZ + v / (kTX * kTY);
         const int64_t vox = ((int64_t)gz * Y + gy) * X + gx;
 #pragma unroll
-        for (int c = 0; c < NCT; ++c) {
-            const int o = c * 16 + 4 * g;
+        for (int c = 0; c < CG; ++c) {
+            const int o = (ct0 + c) * 16 + 4 * g;
             if (o >= COUT) continue;
             uint16_t h[4];
 #pragma unroll
@@ -280,63 +288,103 @@ __global__ __launch_bounds__(256) void bias_grad(const uint16_t* __restrict__ dy
     if (threadIdx.x == 0) db[o] = accumulate ? db[o] + red[0] : red[0];
 }
 
-// 2 x 2 x 2 average pool (out grid X x Y x Z, input 2X x 2Y x 2Z)
+// 8 bf16 <-> 8 floats
+__device__ __forceinline__ void unpack8(uint4 u, float (&f)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        f[2 * q] = __uint_as_float(w[q] << 16);
+        f[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+    }
+}
+
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = (uint32_t)f2bf(f[2 * q]) | ((uint32_t)f2bf(f[2 * q + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 2 x 2 x 2 average pool (out grid X x Y x Z, input 2X x 2Y x 2Z); one thread
+// per 8 channels of a coarse voxel
 __global__ void pool2(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int X, int Y, int Z, int C) {
+    const int C8 = C / 8;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)X * Y * Z * C) return;
-    const int c = (int)(i % C);
-    const int64_t v = i / C;
+    if (i >= (int64_t)X * Y * Z * C8) return;
+    const int c8 = (int)(i % C8);
+    const int64_t v = i / C8;
     const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
-    float s = 0.f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int d = 0; d < 8; ++d) {
         const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
-        s += bf2f(in[(((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c]);
+        float f[8];
+        unpack8(*reinterpret_cast<const uint4*>(in + (((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c8 * 8), f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] += f[e];
     }
-    out[i] = f2bf(s * 0.125f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] *= 0.125f;
+    *reinterpret_cast<uint4*>(out + v * C + c8 * 8) = pack8(s);
 }
 
-// backward of pool2: every fine voxel gets dy / 8
+// backward of pool2: every fine voxel gets dy / 8 (one thread per 8 channels
+// of a fine voxel)
 __global__ void pool2_back(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int X, int Y, int Z, int C) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // fine element
-    const int64_t n = (int64_t)8 * X * Y * Z * C;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const int64_t v = i / C;
+    const int C8 = C / 8;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)8 * X * Y * Z * C8) return;
+    const int c8 = (int)(i % C8);
+    const int64_t v = i / C8;
     const int xi = (int)(v % (2 * X)), yi = (int)((v / (2 * X)) % (2 * Y)), zi = (int)(v / ((int64_t)4 * X * Y));
-    dx[i] = f2bf(bf2f(dy[(((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c]) * 0.125f);
+    float f[8];
+    unpack8(*reinterpret_cast<const uint4*>(dy + (((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c8 * 8), f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) f[e] *= 0.125f;
+    *reinterpret_cast<uint4*>(dx + v * C + c8 * 8) = pack8(f);
 }
 
-// nearest x2 upsample (out grid 2X x 2Y x 2Z)
+// nearest x2 upsample (out grid 2X x 2Y x 2Z), 8 channels per thread
 __global__ void up2(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int X, int Y, int Z, int C) {
+    const int C8 = C / 8;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)8 * X * Y * Z * C;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const int64_t v = i / C;
+    if (i >= (int64_t)8 * X * Y * Z * C8) return;
+    const int c8 = (int)(i % C8);
+    const int64_t v = i / C8;
     const int xi = (int)(v % (2 * X)), yi = (int)((v / (2 * X)) % (2 * Y)), zi = (int)(v / ((int64_t)4 * X * Y));
-    out[i] = in[(((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c];
+    *reinterpret_cast<uint4*>(out + v * C + c8 * 8) =
+        *reinterpret_cast<const uint4*>(in + (((int64_t)(zi / 2) * Y + yi / 2) * X + xi / 2) * C + c8 * 8);
 }
 
-// backward of up2: sum of the 8 fine gradients
+// backward of up2: sum of the 8 fine gradients, 8 channels per thread
 __global__ void up2_back(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int X, int Y, int Z, int C) {
+    const int C8 = C / 8;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)X * Y * Z * C) return;
-    const int c = (int)(i % C);
-    const int64_t v = i / C;
+    if (i >= (int64_t)X * Y * Z * C8) return;
+    const int c8 = (int)(i % C8);
+    const int64_t v = i / C8;
     const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
-    float s = 0.f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int d = 0; d < 8; ++d) {
         const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
-        s += bf2f(dy[(((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c]);
+        float f[8];
+        unpack8(*reinterpret_cast<const uint4*>(dy + (((int64_t)zi * 2 * Y + yi) * 2 * X + xi) * C + c8 * 8), f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] += f[e];
     }
-    dx[i] = f2bf(s);
+    *reinterpret_cast<uint4*>(dx + v * C + c8 * 8) = pack8(s);
 }
 
-// out = a + b (bf16); with accumulate, out += a
+// out = a + b (bf16), 8 elements per thread (n a multiple of 8)
 __global__ void add_bf(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, uint16_t* __restrict__ out,
                        int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+    if (i >= n / 8) return;
+    float fa[8], fb[8];
+    unpack8(reinterpret_cast<const uint4*>(a)[i], fa);
+    unpack8(reinterpret_cast<const uint4*>(b)[i], fb);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) fa[e] = fa[e] + fb[e];
+    reinterpret_cast<uint4*>(out)[i] = pack8(fa);
 }
 
 // Input features on the half-resolution grid (X = N / 2), 24 channels per
@@ -371,17 +419,28 @@ __global__ void features(const float* __restrict__ p, const float* __restrict__ 
 
 // Full-resolution output: out[fine] = p[fine] + out_scale * head[coarse][sub],
 // sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1); the two outer layers, which every
-// substep resets, keep p.
+// substep resets, keep p. One thread per coarse voxel: its 8 head values in
+// one 16-byte load, its fine voxels as 4 x-pairs.
 __global__ void head_out(const float* __restrict__ p, const uint16_t* __restrict__ head, int N, float out_scale,
                          float bnd, float* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)N * N * N) return;
-    const int xi = (int)(i % N), yi = (int)((i / N) % N), zi = (int)(i / ((int64_t)N * N));
     const int X = N / 2;
-    const int sub = (zi & 1) * 4 + (yi & 1) * 2 + (xi & 1);
-    const float d = bf2f(head[(((int64_t)(zi / 2) * X + yi / 2) * X + xi / 2) * 8 + sub]);
-    const bool edge = xi < 2 || yi < 2 || zi < 2 || xi >= N - 2 || yi >= N - 2 || zi >= N - 2;
-    out[i] = edge ? p[i] : p[i] + out_scale * d;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (int64_t)X * X * X) return;
+    const int x = (int)(c % X), y = (int)((c / X) % X), z = (int)(c / ((int64_t)X * X));
+    float h[8];
+    unpack8(*reinterpret_cast<const uint4*>(head + c * 8), h);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int yi = 2 * y + (q & 1), zi = 2 * z + (q >> 1);
+        const int64_t o = ((int64_t)zi * N + yi) * N + 2 * x;
+        const float2 pv = *reinterpret_cast<const float2*>(p + o);
+        const bool row_edge = yi < 2 || zi < 2 || yi >= N - 2 || zi >= N - 2;
+        const int x0 = 2 * x, x1 = 2 * x + 1;
+        float2 r;
+        r.x = (row_edge || x0 < 2 || x0 >= N - 2) ? pv.x : pv.x + out_scale * h[q * 2];
+        r.y = (row_edge || x1 < 2 || x1 >= N - 2) ? pv.y : pv.y + out_scale * h[q * 2 + 1];
+        *reinterpret_cast<float2*>(out + o) = r;
+    }
 }
 
 // Loss = mean over interior fine voxels of (pred - target)^2 / scale^2, with
@@ -456,9 +515,15 @@ template <int CIN, int COUT>
 hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
                   hipStream_t st) {
     constexpr int TZ = tz_for(CIN);
+    constexpr int NCT = cpad(COUT) / 16;
     if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ>), dim3(X / kTX, Y / kTY, Z / TZ), dim3(kThreads), 0, st, x, w, b, y,
-                       X, Y, Z, relu);
+    const int64_t base = (int64_t)(X / kTX) * (Y / kTY) * (Z / TZ);
+    if (base >= 256 || NCT == 1)
+        hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ, NCT>), dim3(X / kTX, Y / kTY, Z / TZ), dim3(kThreads), 0, st, x,
+                           w, b, y, X, Y, Z, relu);
+    else
+        hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ, 1>), dim3(X / kTX, Y / kTY, (Z / TZ) * NCT), dim3(kThreads), 0,
+                           st, x, w, b, y, X, Y, Z, relu);
     return hipGetLastError();
 }
 
@@ -533,27 +598,28 @@ hipError_t bias_back(const uint16_t* dy, int64_t nvox, int cout, float* db, int 
 }
 
 hipError_t pool(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int C, hipStream_t st) {
-    hipLaunchKernelGGL(pool2, dim3(blocks_for((int64_t)X * Y * Z * C)), dim3(256), 0, st, in, out, X, Y, Z, C);
+    hipLaunchKernelGGL(pool2, dim3(blocks_for((int64_t)X * Y * Z * (C / 8))), dim3(256), 0, st, in, out, X, Y, Z, C);
     return hipGetLastError();
 }
 
 hipError_t pool_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st) {
-    hipLaunchKernelGGL(pool2_back, dim3(blocks_for((int64_t)8 * X * Y * Z * C)), dim3(256), 0, st, dy, dx, X, Y, Z, C);
+    hipLaunchKernelGGL(pool2_back, dim3(blocks_for((int64_t)8 * X * Y * Z * (C / 8))), dim3(256), 0, st, dy, dx, X, Y, Z,
+                       C);
     return hipGetLastError();
 }
 
 hipError_t upsample(const uint16_t* in, uint16_t* out, int X, int Y, int Z, int C, hipStream_t st) {
-    hipLaunchKernelGGL(up2, dim3(blocks_for((int64_t)8 * X * Y * Z * C)), dim3(256), 0, st, in, out, X, Y, Z, C);
+    hipLaunchKernelGGL(up2, dim3(blocks_for((int64_t)8 * X * Y * Z * (C / 8))), dim3(256), 0, st, in, out, X, Y, Z, C);
     return hipGetLastError();
 }
 
 hipError_t upsample_back(const uint16_t* dy, uint16_t* dx, int X, int Y, int Z, int C, hipStream_t st) {
-    hipLaunchKernelGGL(up2_back, dim3(blocks_for((int64_t)X * Y * Z * C)), dim3(256), 0, st, dy, dx, X, Y, Z, C);
+    hipLaunchKernelGGL(up2_back, dim3(blocks_for((int64_t)X * Y * Z * (C / 8))), dim3(256), 0, st, dy, dx, X, Y, Z, C);
     return hipGetLastError();
 }
 
 hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, hipStream_t st) {
-    hipLaunchKernelGGL(add_bf, dim3(blocks_for(n)), dim3(256), 0, st, a, b, out, n);
+    hipLaunchKernelGGL(add_bf, dim3(blocks_for(n / 8)), dim3(256), 0, st, a, b, out, n);
     return hipGetLastError();
 }
 
@@ -566,8 +632,8 @@ hipError_t input_features(const float* p, const float* rate, int N, float bnd, f
 
 hipError_t output_field(const float* p, const uint16_t* head, int N, float out_scale, float bnd, float* out,
                         hipStream_t st) {
-    hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)N * N * N)), dim3(256), 0, st, p, head, N, out_scale, bnd,
-                       out);
+    hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)(N / 2) * (N / 2) * (N / 2))), dim3(256), 0, st, p, head, N,
+                       out_scale, bnd, out);
     return hipGetLastError();
 }
 
