@@ -81,6 +81,14 @@ struct PhaseScope {
 
 void release(gell_ctx* c) {
     if (!c) return;
+    for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) {
+        if (c->sub_stream[s]) {
+            (void)hipStreamSynchronize(c->sub_stream[s]);
+            (void)hipStreamDestroy(c->sub_stream[s]);
+        }
+        if (c->sub_fork[s]) (void)hipEventDestroy(c->sub_fork[s]);
+        if (c->sub_join[s]) (void)hipEventDestroy(c->sub_join[s]);
+    }
     for (auto& m : c->marks) {
         (void)hipEventDestroy(m.a);
         (void)hipEventDestroy(m.b);
@@ -226,6 +234,11 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->compact, cap));
     if (P.N == 128 || P.N == 256 || P.N == 512) A(dalloc(&c->coup, (size_t)(4 * (P.N / 64)) * (size_t)P.N * P.N));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_total), sizeof(int32_t));
+    for (int s = 1; s < P.n_substrates; ++s) {
+        A(hipStreamCreateWithFlags(&c->sub_stream[s], hipStreamNonBlocking));
+        A(hipEventCreateWithFlags(&c->sub_fork[s], hipEventDisableTiming));
+        A(hipEventCreateWithFlags(&c->sub_join[s], hipEventDisableTiming));
+    }
     if (e != hipSuccess) {
         release(c);
         delete c;
@@ -253,6 +266,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
 
 void gell_destroy(gell_ctx* ctx) {
     if (!ctx) return;
+    (void)gell::join_substrates(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     release(ctx);
     delete ctx;
@@ -291,10 +305,17 @@ int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int3
     if (substrate < 0 || substrate >= ctx->P.n_substrates) return fail(ctx, GELL_E_ARG, "substrate out of range");
     int r = check_cells(ctx, d_cells, n);
     if (r) return r;
-    hipError_t e;
+    // the substrate's previous substeps read the uptake map about to be rewritten
+    hipError_t e = gell::join_substrate(ctx, substrate);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prelod");
     {
         PhaseScope ps(ctx, gell::PH_PRELOD);
         e = gell::prelod_launch(ctx, substrate, d_cells, n);
+    }
+    // fork point of the substrate's own stream: right after its uptake map
+    if (e == hipSuccess && gell::substrate_stream_on(ctx, substrate)) {
+        e = hipEventRecord(ctx->sub_fork[substrate], ctx->stream);
+        ctx->sub_forked[substrate] = e == hipSuccess;
     }
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "prelod");
 }
@@ -302,10 +323,23 @@ int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int3
 int gell_lod_substep(gell_ctx* ctx, int32_t substrate) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
     if (substrate < 0 || substrate >= ctx->P.n_substrates) return fail(ctx, GELL_E_ARG, "substrate out of range");
-    hipError_t e;
+    hipError_t e = hipSuccess;
+    hipStream_t st = ctx->stream;
+    if (gell::substrate_stream_on(ctx, substrate)) {
+        if (!ctx->sub_pending[substrate]) {
+            if (!ctx->sub_forked[substrate]) e = hipEventRecord(ctx->sub_fork[substrate], ctx->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ctx->sub_stream[substrate], ctx->sub_fork[substrate], 0);
+            ctx->sub_forked[substrate] = false;
+            ctx->sub_pending[substrate] = e == hipSuccess;
+        }
+        st = ctx->sub_stream[substrate];
+    } else {
+        e = gell::join_substrate(ctx, substrate);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "lod substep");
     {
         PhaseScope ps(ctx, gell::PH_LOD);
-        e = gell::lod_substep_launch(ctx, substrate);
+        e = gell::lod_substep_launch(ctx, substrate, st);
     }
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "lod substep");
 }
@@ -357,7 +391,8 @@ const int32_t* gell_birth_slots(gell_ctx* ctx) { return ctx ? ctx->slots : nullp
 
 int gell_sync(gell_ctx* ctx) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
-    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = gell::join_substrates(ctx);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "sync");
 }
 
@@ -374,13 +409,16 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
                           : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && (value == 2 || value == 3)));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
+    const hipError_t e = gell::join_substrates(ctx);
+    if (e != hipSuccess) return hip_fail(ctx, e, "tune");
     ctx->tune[knob] = value;
     return GELL_OK;
 }
 
 int gell_autotune(gell_ctx* ctx, int flags, double* lod_us_out) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
-    hipError_t e = gell::lod_autotune(ctx, (flags & GELL_TUNE_BIT_EXACT) != 0, lod_us_out);
+    hipError_t e = gell::join_substrates(ctx);
+    if (e == hipSuccess) e = gell::lod_autotune(ctx, (flags & GELL_TUNE_BIT_EXACT) != 0, lod_us_out);
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "autotune");
 }
 
@@ -391,6 +429,8 @@ int gell_tune_get(const gell_ctx* ctx, int knob) {
 
 int gell_profile_enable(gell_ctx* ctx, int enable) {
     if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
+    const hipError_t e = gell::join_substrates(ctx);
+    if (e != hipSuccess) return hip_fail(ctx, e, "profile");
     ctx->prof = enable != 0;
     return GELL_OK;
 }
@@ -446,3 +486,25 @@ int gell_device_name(char* buf, int len) {
 }
 
 }  // extern "C"
+
+namespace gell {
+
+bool substrate_stream_on(const gell_ctx* ctx, int s) {
+    return s >= 1 && ctx->sub_stream[s] && ctx->tune[GELL_KNOB_SUBSTRATE_STREAMS] && !ctx->prof;
+}
+
+hipError_t join_substrate(gell_ctx* ctx, int s) {
+    if (s < 1 || s >= GELL_MAX_SUBSTRATES || !ctx->sub_pending[s]) return hipSuccess;
+    hipError_t e = hipEventRecord(ctx->sub_join[s], ctx->sub_stream[s]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->sub_join[s], 0);
+    if (e == hipSuccess) ctx->sub_pending[s] = false;
+    return e;
+}
+
+hipError_t join_substrates(gell_ctx* ctx) {
+    hipError_t e = hipSuccess;
+    for (int s = 1; s < ctx->P.n_substrates && e == hipSuccess; ++s) e = join_substrate(ctx, s);
+    return e;
+}
+
+}  // namespace gell

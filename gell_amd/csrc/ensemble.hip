@@ -82,6 +82,9 @@ int member_step(gell_ensemble* ens, int r, gell_cell* cells, int32_t n) {
     for (int s = 0; rc == GELL_OK && s < c->P.n_substrates; ++s)
         for (int d = 0; rc == GELL_OK && d < ens->diff_steps; ++d) rc = gell_lod_substep(c, s);
     if (rc != GELL_OK) return ens_fail(ens, rc, "replicate " + std::to_string(r) + ": " + c->err);
+    // the replicate's own substrate streams join its stream before the join event
+    const hipError_t e = gell::join_substrates(c);
+    if (e != hipSuccess) return ens_hip(ens, e, "substrate join");
     return GELL_OK;
 }
 

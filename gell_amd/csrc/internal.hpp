@@ -54,7 +54,17 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 3, 4, 4, 1};
+    int tune[GELL_N_KNOBS] = {1, 3, 4, 4, 1, 1};
+
+    // Substrates >= 1 run their LOD substeps on streams of their own when
+    // GELL_KNOB_SUBSTRATE_STREAMS is set: forked from the context stream after
+    // their uptake map (or at their first substep), joined back before
+    // anything else touches that substrate (see gell::join_substrate).
+    hipStream_t sub_stream[GELL_MAX_SUBSTRATES] = {};
+    hipEvent_t sub_fork[GELL_MAX_SUBSTRATES] = {};
+    hipEvent_t sub_join[GELL_MAX_SUBSTRATES] = {};
+    bool sub_forked[GELL_MAX_SUBSTRATES] = {};   // fork event recorded, not yet waited on
+    bool sub_pending[GELL_MAX_SUBSTRATES] = {};  // work on sub_stream not yet joined
     float* coup = nullptr;  // y-coupling values of the fused LOD substep (4 (N / 64) N^2 floats)
 
     // Profiling.
@@ -74,7 +84,13 @@ namespace gell {
 enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, PH_LOD, PH_BIRTH, PH_DEATH };
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
-hipError_t lod_substep_launch(gell_ctx* ctx, int s);
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);
+// Make the context stream wait for substrate s's own stream (no-op when
+// nothing is pending there); join_substrates does it for every substrate.
+hipError_t join_substrate(gell_ctx* ctx, int s);
+// Whether substrate s's substeps go to its own stream now.
+bool substrate_stream_on(const gell_ctx* ctx, int s);
+hipError_t join_substrates(gell_ctx* ctx);
 // Whether the next substep takes the fused path (knobs and grid allow it).
 bool lod_fused_active(const gell_ctx* ctx);
 hipError_t lod_init_coeffs(gell_ctx* ctx, int s);

@@ -649,14 +649,13 @@ bool lod_fused_active(const gell_ctx* ctx) {
            (N == 128 || N == 256 || N == 512);
 }
 
-hipError_t lod_substep_launch(gell_ctx* ctx, int s) {
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     const int N = ctx->N;
     float* p = ctx->field[s];
     const float* rate = ctx->rate[s];
     const float* E = ctx->E[s];
     const float* F = ctx->F[s];
     const float side = ctx->side[s], bnd = ctx->P.sub[s].boundary, dt = ctx->P.dt_diff;
-    hipStream_t st = ctx->stream;
     const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
     const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
@@ -825,10 +824,10 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
         if (e != hipSuccess) break;
         const int val[4] = {c.xp, c.yzp, c.sp, c.fused};
         for (int q = 0; q < 4; ++q) ctx->tune[knobs[q]] = val[q];
-        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = lod_substep_launch(ctx, 0);
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = lod_substep_launch(ctx, 0, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
         const int reps = 5;
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = lod_substep_launch(ctx, 0);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = lod_substep_launch(ctx, 0, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float ms = 0.f;

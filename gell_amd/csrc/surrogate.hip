@@ -459,13 +459,14 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     hipStream_t st = s->stream;
     const int64_t V = ctx->V;
     float* const field = ctx->field[sub];
-    hipError_t e = forward(s, field, ctx->rate[sub]);
+    hipError_t e = gell::join_substrates(ctx);
+    if (e == hipSuccess) e = forward(s, field, ctx->rate[sub]);
     if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, s->pred, st);
     // one more substep from the prediction: its relative change is the residual
     if (e == hipSuccess) e = hipMemcpyAsync(s->check, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess) {
         ctx->field[sub] = s->check;
-        e = gell::lod_substep_launch(ctx, sub);
+        e = gell::lod_substep_launch(ctx, sub, st);
         ctx->field[sub] = field;
     }
     if (e == hipSuccess) hipLaunchKernelGGL(diff_norms, dim3(kRedBlocks), dim3(256), 0, st, s->check, s->pred, V, s->red_part);
@@ -484,7 +485,7 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     if (*used) {
         e = hipMemcpyAsync(field, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
     } else {
-        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub);
+        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
     }
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
 }

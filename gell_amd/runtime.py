@@ -357,7 +357,8 @@ class Context:
         return download_ptr(self.lib.gell_birth_slots(self.h), np.int32, n)
 
     # -- kernel variants -------------------------------------------------------
-    KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1, "lod_parts": 2, "lod_x_parts": 3, "lod_fused": 4}
+    KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1, "lod_parts": 2, "lod_x_parts": 3, "lod_fused": 4,
+             "substrate_streams": 5}
 
     def tune(self, knob: str, value: int) -> None:
         self._check(self.lib.gell_tune(self.h, self.KNOBS[knob], int(value)), "gell_tune")

@@ -191,6 +191,12 @@ int gell_sync(gell_ctx* ctx);
  *                              N / 64 blocks, z in LOD_PARTS blocks; results
  *                              equal the unfused kernels with those block
  *                              counts bit for bit. Default 1.
+ *   GELL_KNOB_SUBSTRATE_STREAMS 1 = the LOD substeps of substrates >= 1 run
+ *                              on streams of their own, overlapping the
+ *                              other substrates' substeps (same results;
+ *                              off while profiling). Field and uptake-map
+ *                              pointers are current after gell_sync.
+ *                              Default 1.
  * The first two knobs never change results; gell_autotune sets all of them. */
 enum {
     GELL_KNOB_LOD_YZ_LDS = 0,
@@ -198,7 +204,8 @@ enum {
     GELL_KNOB_LOD_PARTS = 2,
     GELL_KNOB_LOD_X_PARTS = 3,
     GELL_KNOB_LOD_FUSED = 4,
-    GELL_N_KNOBS = 5
+    GELL_KNOB_SUBSTRATE_STREAMS = 5,
+    GELL_N_KNOBS = 6
 };
 int gell_tune(gell_ctx* ctx, int knob, int value);
 /* Time every effective LOD variant on a scratch copy of substrate 0's field
