@@ -121,7 +121,7 @@ void release(gell_ctx* c) {
         dfree(c->acc[s]);
         dfree(c->rate_list[s]);
     }
-    dfree(c->coup);
+    for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) dfree(c->coup[s]);
     dfree(c->flags);
     dfree(c->blk);
     dfree(c->slots);
@@ -232,9 +232,10 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->slots, cap));
     A(dalloc(&c->d_total, 1));
     A(dalloc(&c->compact, cap));
-    if (P.N == 128 || P.N == 256 || P.N == 512) A(dalloc(&c->coup, (size_t)(4 * (P.N / 64)) * (size_t)P.N * P.N));
+    if (P.N == 128 || P.N == 256 || P.N == 512)
+        for (int s = 0; s < P.n_substrates; ++s) A(dalloc(&c->coup[s], (size_t)(4 * (P.N / 64)) * (size_t)P.N * P.N));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_total), sizeof(int32_t));
-    for (int s = 1; s < P.n_substrates; ++s) {
+    for (int s = 0; s < P.n_substrates; ++s) {
         A(hipStreamCreateWithFlags(&c->sub_stream[s], hipStreamNonBlocking));
         A(hipEventCreateWithFlags(&c->sub_fork[s], hipEventDisableTiming));
         A(hipEventCreateWithFlags(&c->sub_join[s], hipEventDisableTiming));
@@ -348,7 +349,9 @@ int gell_birth(gell_ctx* ctx, gell_cell* d_cells, int32_t n, uint64_t bio_step, 
     if (!ctx || !n_out) return fail(ctx, GELL_E_ARG, "null argument");
     int r = check_cells(ctx, d_cells, n);
     if (r) return r;
-    hipError_t e;
+    // birth reads substrate 0 (oxygen), whose substeps may still run on its stream
+    hipError_t e = gell::join_substrate(ctx, 0);
+    if (e != hipSuccess) return hip_fail(ctx, e, "birth");
     {
         PhaseScope ps(ctx, gell::PH_BIRTH);
         e = gell::birth_launch(ctx, d_cells, n, bio_step, n_out);
@@ -490,11 +493,11 @@ int gell_device_name(char* buf, int len) {
 namespace gell {
 
 bool substrate_stream_on(const gell_ctx* ctx, int s) {
-    return s >= 1 && ctx->sub_stream[s] && ctx->tune[GELL_KNOB_SUBSTRATE_STREAMS] && !ctx->prof;
+    return s >= 0 && ctx->sub_stream[s] && ctx->tune[GELL_KNOB_SUBSTRATE_STREAMS] && !ctx->prof;
 }
 
 hipError_t join_substrate(gell_ctx* ctx, int s) {
-    if (s < 1 || s >= GELL_MAX_SUBSTRATES || !ctx->sub_pending[s]) return hipSuccess;
+    if (s < 0 || s >= GELL_MAX_SUBSTRATES || !ctx->sub_pending[s]) return hipSuccess;
     hipError_t e = hipEventRecord(ctx->sub_join[s], ctx->sub_stream[s]);
     if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->sub_join[s], 0);
     if (e == hipSuccess) ctx->sub_pending[s] = false;
@@ -503,7 +506,7 @@ hipError_t join_substrate(gell_ctx* ctx, int s) {
 
 hipError_t join_substrates(gell_ctx* ctx) {
     hipError_t e = hipSuccess;
-    for (int s = 1; s < ctx->P.n_substrates && e == hipSuccess; ++s) e = join_substrate(ctx, s);
+    for (int s = 0; s < ctx->P.n_substrates && e == hipSuccess; ++s) e = join_substrate(ctx, s);
     return e;
 }
 

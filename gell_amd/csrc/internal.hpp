@@ -56,16 +56,19 @@ struct gell_ctx {
     // Kernel-variant knobs (GELL_KNOB_*).
     int tune[GELL_N_KNOBS] = {1, 3, 4, 4, 1, 1};
 
-    // Substrates >= 1 run their LOD substeps on streams of their own when
+    // Every substrate runs its LOD substeps on a stream of its own when
     // GELL_KNOB_SUBSTRATE_STREAMS is set: forked from the context stream after
-    // their uptake map (or at their first substep), joined back before
-    // anything else touches that substrate (see gell::join_substrate).
+    // its uptake map (or at its first substep), joined back before anything
+    // else touches that substrate (see gell::join_substrate). The mechanics
+    // of the next step, which read cells only, overlap them.
     hipStream_t sub_stream[GELL_MAX_SUBSTRATES] = {};
     hipEvent_t sub_fork[GELL_MAX_SUBSTRATES] = {};
     hipEvent_t sub_join[GELL_MAX_SUBSTRATES] = {};
     bool sub_forked[GELL_MAX_SUBSTRATES] = {};   // fork event recorded, not yet waited on
     bool sub_pending[GELL_MAX_SUBSTRATES] = {};  // work on sub_stream not yet joined
-    float* coup = nullptr;  // y-coupling values of the fused LOD substep (4 (N / 64) N^2 floats)
+    // y-coupling values of the fused LOD substep, one buffer per substrate
+    // (substrates may run concurrently): 4 (N / 64) N^2 floats each
+    float* coup[GELL_MAX_SUBSTRATES] = {};
 
     // Profiling.
     bool prof = false;

@@ -645,7 +645,7 @@ hipError_t launch_fused(float* p, const float* rate, float* const* tabs, float s
 
 bool lod_fused_active(const gell_ctx* ctx) {
     const int N = ctx->N;
-    return ctx->tune[GELL_KNOB_LOD_FUSED] && ctx->tune[GELL_KNOB_LOD_PARTS] > 1 && ctx->coup &&
+    return ctx->tune[GELL_KNOB_LOD_FUSED] && ctx->tune[GELL_KNOB_LOD_PARTS] > 1 && ctx->coup[0] &&
            (N == 128 || N == 256 || N == 512);
 }
 
@@ -661,9 +661,9 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
     if (lod_fused_active(ctx)) {
         float* const* tabs = ctx->ptab[s];
-        if (N == 128) return launch_fused<128>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
-        if (N == 256) return launch_fused<256>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
-        return launch_fused<512>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup, st);
+        if (N == 128) return launch_fused<128>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 256) return launch_fused<256>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        return launch_fused<512>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
     }
     if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256 || N == 512)) {
         float* const* tabs = ctx->ptab[s];
@@ -812,7 +812,7 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
                 if ((xp > 1 && N % (xp * 32) != 0) || (yzp > 1 && N % (yzp * 32) != 0)) continue;
                 cands.push_back({xp, yzp, sp, 0});
             }
-    if (part_ok && ctx->coup)
+    if (part_ok && ctx->coup[0])
         for (int yi = 1; yi < 5; ++yi)
             if (N % (cand[yi] * 32) == 0) cands.push_back({N / 64, cand[yi], 0, 1});
     const int knobs[4] = {GELL_KNOB_LOD_X_PARTS, GELL_KNOB_LOD_PARTS, GELL_KNOB_LOD_YZ_LDS, GELL_KNOB_LOD_FUSED};

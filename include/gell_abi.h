@@ -191,12 +191,12 @@ int gell_sync(gell_ctx* ctx);
  *                              N / 64 blocks, z in LOD_PARTS blocks; results
  *                              equal the unfused kernels with those block
  *                              counts bit for bit. Default 1.
- *   GELL_KNOB_SUBSTRATE_STREAMS 1 = the LOD substeps of substrates >= 1 run
- *                              on streams of their own, overlapping the
- *                              other substrates' substeps (same results;
- *                              off while profiling). Field and uptake-map
- *                              pointers are current after gell_sync.
- *                              Default 1.
+ *   GELL_KNOB_SUBSTRATE_STREAMS 1 = each substrate's LOD substeps run on a
+ *                              stream of its own, overlapping the other
+ *                              substrates' substeps and the next mechanics
+ *                              step (same results; off while profiling).
+ *                              Field and uptake-map pointers are current
+ *                              after gell_sync. Default 1.
  * The first two knobs never change results; gell_autotune sets all of them. */
 enum {
     GELL_KNOB_LOD_YZ_LDS = 0,
