@@ -109,6 +109,8 @@ void release(gell_ctx* c) {
     dfree(c->se);
     dfree(c->se_lin);
     dfree(c->se_list);
+    dfree(c->mblk);
+    dfree(c->n_out);
     for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) {
         dfree(c->field[s]);
         dfree(c->rate[s]);
@@ -215,6 +217,9 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->nb, cap));
     A(dalloc(&c->se, (size_t)c->KS));
     A(dalloc(&c->se_list, cap));
+    c->nmblk = c->KS >= 512 ? c->KS / 512 : 1;
+    A(dalloc(&c->mblk, (size_t)c->nmblk));
+    A(dalloc(&c->n_out, 1));
     for (int s = 0; s < P.n_substrates; ++s) {
         A(dalloc(&c->field[s], (size_t)c->V));
         A(dalloc(&c->rate[s], (size_t)c->V));
@@ -410,7 +415,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
     const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
-                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && (value == 2 || value == 3)));
+                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 5));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     const hipError_t e = gell::join_substrates(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "tune");

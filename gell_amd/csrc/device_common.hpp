@@ -88,6 +88,16 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
     return v;
 }
 
+// Inverse of spread3: every third bit of v (bit 3b -> bit b), 10 bits.
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0x030000FFu;
+    v = (v | (v >> 16)) & 0x3FFu;
+    return v;
+}
+
 // Morton key, x in bit 0 (BasicFunctions.cuh:129-137).
 __device__ __forceinline__ uint32_t morton3(int32_t i, int32_t j, int32_t k) {
     return spread3((uint32_t)i) | (spread3((uint32_t)j) << 1) | (spread3((uint32_t)k) << 2);

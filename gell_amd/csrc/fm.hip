@@ -283,9 +283,25 @@ const PassFn kPass[10] = {nullptr,         sort_pass<1>, sort_pass<2>, sort_pass
 __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict__ cells,
                                                     const int32_t* __restrict__ order, int32_t n, int32_t N, float dx,
                                                     gell_cell* __restrict__ sorted, float4* __restrict__ nb,
-                                                    int2* __restrict__ se, int32_t* __restrict__ se_list) {
+                                                    int2* __restrict__ se, int32_t* __restrict__ se_list,
+                                                    const int32_t* __restrict__ skeys, int2* __restrict__ blk,
+                                                    int32_t* __restrict__ n_out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (blk) {
+        // Boundaries of the aligned 8^3 Morton blocks in sorted order (keys are
+        // sorted, so each boundary has exactly one writer); the outside
+        // cells (key -1) sort first and their count goes to *n_out.
+        const auto block_of = [&](int64_t q) { return skeys[q] >= 0 ? skeys[q] >> 9 : -1; };
+        const int32_t b = block_of(i);
+        const int32_t bp = i > 0 ? block_of(i - 1) : -2, bn = i + 1 < n ? block_of(i + 1) : -2;
+        if (b >= 0) {
+            if (bp != b) blk[b].x = (int32_t)i;
+            if (bn != b) blk[b].y = (int32_t)i;
+        } else if (bn != b) {
+            *n_out = (int32_t)i + 1;
+        }
+    }
     CellRegs c = load_cell(cells, order[i]);
     const float px = cf(c, W_PX), py = cf(c, W_PY), pz = cf(c, W_PZ);
     const int32_t a = trunc_index(px / dx), b = trunc_index(py / dx), k = trunc_index(pz / dx);
@@ -479,6 +495,35 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
     }
 }
 
+// movementkernel, FMkernel.cuh:130-158: AB2 step of a live cell; outside the
+// domain the cell is flagged dead and keeps its position.
+__device__ __forceinline__ void move_cell(CellRegs& c, float fx, float fy, float fz, int32_t N, float dx, float dt) {
+    const float p1x = cf(c, W_PX), p1y = cf(c, W_PY), p1z = cf(c, W_PZ);
+    const float h = dt / 2.f;
+    const float ox = cf(c, W_OX), oy = cf(c, W_OY), oz = cf(c, W_OZ);
+    const float nx = p1x + h * (3.f * fx - ox);
+    const float ny = p1y + h * (3.f * fy - oy);
+    const float nz = p1z + h * (3.f * fz - oz);
+    const int32_t a = trunc_index(nx / dx), b = trunc_index(ny / dx), k = trunc_index(nz / dx);
+    if (!inside(a, b, k, N)) {
+        cset_sign(c, false);
+        cset(c, W_FX, fx);
+        cset(c, W_FY, fy);
+        cset(c, W_FZ, fz);
+    } else {
+        cset(c, W_PX, nx);
+        cset(c, W_PY, ny);
+        cset(c, W_PZ, nz);
+        cset(c, W_OX, fx);
+        cset(c, W_OY, fy);
+        cset(c, W_OZ, fz);
+        cset(c, W_FX, 0.f);
+        cset(c, W_FY, 0.f);
+        cset(c, W_FZ, 0.f);
+        c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
+    }
+}
+
 // VARIANT 0: stencil voxels in turn; 1: their ranges loaded up front;
 // 2: flattened per-lane candidate walk; 3: grouped candidate loads. All four
 // give the same bits.
@@ -504,34 +549,150 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
         cell_force<VARIANT == 1>(nb, se, N, dx, cf(c, W_PX), cf(c, W_PY), cf(c, W_PZ), cf(c, W_R), fx, fy, fz);
     }
     if (!in_range) return;
-    if (live) {
-        const float p1x = cf(c, W_PX), p1y = cf(c, W_PY), p1z = cf(c, W_PZ);
-        // movementkernel, FMkernel.cuh:130-158
-        const float h = dt / 2.f;
-        const float ox = cf(c, W_OX), oy = cf(c, W_OY), oz = cf(c, W_OZ);
-        const float nx = p1x + h * (3.f * fx - ox);
-        const float ny = p1y + h * (3.f * fy - oy);
-        const float nz = p1z + h * (3.f * fz - oz);
-        const int32_t a = trunc_index(nx / dx), b = trunc_index(ny / dx), k = trunc_index(nz / dx);
-        if (!inside(a, b, k, N)) {
-            cset_sign(c, false);
-            cset(c, W_FX, fx);
-            cset(c, W_FY, fy);
-            cset(c, W_FZ, fz);
-        } else {
-            cset(c, W_PX, nx);
-            cset(c, W_PY, ny);
-            cset(c, W_PZ, nz);
-            cset(c, W_OX, fx);
-            cset(c, W_OY, fy);
-            cset(c, W_OZ, fz);
-            cset(c, W_FX, 0.f);
-            cset(c, W_FY, 0.f);
-            cset(c, W_FZ, 0.f);
-            c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
-        }
-    }
+    if (live) move_cell(c, fx, fy, fz, N, dx, dt);
     store_cell(cells, i, c);
+}
+
+// Variant 4: the grouped candidate walk of variant 3, with the cell's own
+// (x, y, z, r) taken from the compact neighbour array. The block's records,
+// contiguous in sorted order, are read as coalesced 16-byte chunks together
+// with that first load and staged in LDS; after the force loop each thread
+// applies the movement to its record there, and the block writes the records
+// back as 16-byte chunks. Same bits as variants 0-3. Needs 16-byte aligned
+// record arrays (the launcher falls back to variant 3 otherwise).
+constexpr int kRecDw = (int)(sizeof(gell_cell) / 4);  // 18 dwords
+
+__global__ __launch_bounds__(kBlock) void fm_force_move_staged(const gell_cell* __restrict__ sorted,
+                                                               const float4* __restrict__ nb,
+                                                               const int2* __restrict__ se,
+                                                               gell_cell* __restrict__ cells, int32_t n, int32_t N,
+                                                               float dx, float dt) {
+    __shared__ uint32_t rec[kBlock * kRecDw];
+    const int64_t b0 = (int64_t)blockIdx.x * kBlock;
+    const int m = (int)((int64_t)n - b0 < kBlock ? (int64_t)n - b0 : kBlock);  // records in this block
+    const int ndw = m * kRecDw;
+    const int nq = ndw / 4;  // whole 16-byte chunks; an odd m leaves one 8-byte tail
+    const uint4* src = reinterpret_cast<const uint4*>(sorted + b0);
+    const int64_t i = b0 + threadIdx.x;
+    const bool in_range = (int)threadIdx.x < m;
+    const float4 me = in_range ? nb[i] : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
+    for (int q = threadIdx.x; q < nq; q += kBlock) {
+        const uint4 v = src[q];
+        rec[4 * q] = v.x;
+        rec[4 * q + 1] = v.y;
+        rec[4 * q + 2] = v.z;
+        rec[4 * q + 3] = v.w;
+    }
+    if (threadIdx.x == 0 && (ndw & 3)) {
+        const uint2 v = reinterpret_cast<const uint2*>(sorted + b0)[2 * nq];
+        rec[4 * nq] = v.x;
+        rec[4 * nq + 1] = v.y;
+    }
+    const bool live = in_range && __float_as_uint(me.w) != kDeadTag;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (live) cell_force_grouped(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+    __syncthreads();
+    if (in_range) {
+        CellRegs c;
+#pragma unroll
+        for (int w = 0; w < kRecDw; ++w) c.w[w] = rec[threadIdx.x * kRecDw + w];
+        if (live) move_cell(c, fx, fy, fz, N, dx, dt);
+#pragma unroll
+        for (int w = 0; w < kRecDw; ++w) rec[threadIdx.x * kRecDw + w] = c.w[w];
+    }
+    __syncthreads();
+    uint4* dst = reinterpret_cast<uint4*>(cells + b0);
+    for (int q = threadIdx.x; q < nq; q += kBlock)
+        dst[q] = make_uint4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
+    if (threadIdx.x == 0 && (ndw & 3))
+        reinterpret_cast<uint2*>(cells + b0)[2 * nq] = make_uint2(rec[4 * nq], rec[4 * nq + 1]);
+}
+
+__global__ void blk_reset(int2* __restrict__ blk, int64_t nblk, int32_t* __restrict__ n_out) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblk) blk[b] = make_int2(0x7FFFFFFF, -1);
+    if (b == 0) *n_out = 0;
+}
+
+// Variant 5: one workgroup per aligned 8^3 Morton block. The block's cells
+// are contiguous in sorted order (first / last slot from the gather). The
+// workgroup loads the ranges of the 10^3 halo voxels and then every candidate
+// neighbour of the halo into LDS (each once, all loads independent); each cell
+// then walks its 27 stencil voxels in the reference's order (x outer, z inner,
+// ascending slot) from LDS, so the bits are those of variants 0-4. A block
+// whose halo holds more than kCand candidates walks global memory instead
+// (variant 0's loop). The outside cells, which sort first, are copied as they
+// are by a grid-stride loop of the same launch.
+constexpr int kHalo = 10, kHaloV = 1000, kCand = 2048;
+
+__global__ __launch_bounds__(kBlock) void fm_force_move_blocks(const gell_cell* __restrict__ sorted,
+                                                               const float4* __restrict__ nb,
+                                                               const int2* __restrict__ se,
+                                                               const int2* __restrict__ blk,
+                                                               const int32_t* __restrict__ n_out,
+                                                               gell_cell* __restrict__ cells, int32_t N, float dx,
+                                                               float dt) {
+    __shared__ int ho[kBlock * 4], hc[kBlock * 4];
+    __shared__ float4 cand[kCand];
+    __shared__ int lds4[kBlock / 64];
+    const int32_t nout = *n_out;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nout; i += (int64_t)gridDim.x * kBlock)
+        store_cell(cells, i, load_cell(sorted, i));
+    const int2 range = blk[blockIdx.x];
+    if (range.x > range.y) return;  // empty block (uniform across the workgroup)
+    const uint32_t m0 = (uint32_t)blockIdx.x << 9;
+    const int32_t bx = (int32_t)compact3(m0), by = (int32_t)compact3(m0 >> 1), bz = (int32_t)compact3(m0 >> 2);
+    // halo ranges: thread t owns halo voxels 4t .. 4t + 3 (x fastest, 10 per row)
+    int hsv[4], hcv[4], local = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int h = 4 * threadIdx.x + e;
+        hsv[e] = 0;
+        hcv[e] = 0;
+        if (h < kHaloV) {
+            const int32_t x = bx - 1 + h % kHalo, y = by - 1 + (h / kHalo) % kHalo, z = bz - 1 + h / (kHalo * kHalo);
+            if (inside(x, y, z, N)) {
+                const int2 r = se[morton3(x, y, z)];
+                hsv[e] = r.x;
+                hcv[e] = r.y >= r.x ? r.y - r.x + 1 : 0;
+            }
+        }
+        local += hcv[e];
+    }
+    int total;
+    int off = block_exclusive_scan(local, lds4, total);
+    const bool fits = total <= kCand;  // uniform
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        ho[4 * threadIdx.x + e] = off;
+        hc[4 * threadIdx.x + e] = hcv[e];
+        if (fits)
+            for (int k = 0; k < hcv[e]; ++k) cand[off + k] = nb[hsv[e] + k];
+        off += hcv[e];
+    }
+    __syncthreads();
+    for (int64_t i = range.x + threadIdx.x; i <= range.y; i += kBlock) {
+        CellRegs c = load_cell(sorted, i);
+        const float4 me = nb[i];
+        if (__float_as_uint(me.w) != kDeadTag) {
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            if (fits) {
+                // FMkernel.cuh:56-118 from the LDS copy
+                const int32_t lx = trunc_index(me.x / dx) - bx, ly = trunc_index(me.y / dx) - by,
+                              lz = trunc_index(me.z / dx) - bz;
+                float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+                for (int q = 0; q < 27; ++q) {
+                    const int h = ((lz + q % 3) * kHalo + (ly + (q / 3) % 3)) * kHalo + (lx + q / 9);
+                    const int k0 = ho[h], k1 = k0 + hc[h];
+                    for (int k = k0; k < k1; ++k) force_pair(cand[k], me.x, me.y, me.z, me.w, thres, fx, fy, fz);
+                }
+            } else {
+                cell_force<false>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+            }
+            move_cell(c, fx, fy, fz, N, dx, dt);
+        }
+        store_cell(cells, i, c);
+    }
 }
 
 __global__ void se_fill(int2* se, int64_t V, int32_t s_empty) {
@@ -602,8 +763,13 @@ hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
 
 hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     if (n == 0) return hipSuccess;
+    const bool blocks = ctx->tune[GELL_KNOB_FM_PREFETCH] == 5;
+    if (blocks)
+        hipLaunchKernelGGL(blk_reset, dim3((unsigned)((ctx->nmblk + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, ctx->mblk, ctx->nmblk, ctx->n_out);
     hipLaunchKernelGGL(fm_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
-                       ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ctx->se, ctx->se_list);
+                       ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ctx->se, ctx->se_list,
+                       ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out);
     ctx->se_list_n = n;
     return hipGetLastError();
 }
@@ -611,8 +777,18 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kBlock - 1) / kBlock);
-    const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
-    if (v == 3)
+    int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    if (v == 5) {
+        hipLaunchKernelGGL(fm_force_move_blocks, dim3((unsigned)ctx->nmblk), dim3(kBlock), 0, ctx->stream,
+                           ctx->sorted, ctx->nb, ctx->se, ctx->mblk, ctx->n_out, cells, ctx->N, ctx->P.dx,
+                           ctx->P.dt_mech);
+        return hipGetLastError();
+    }
+    if (v == 4 && ((reinterpret_cast<uintptr_t>(cells) | reinterpret_cast<uintptr_t>(ctx->sorted)) & 15) != 0) v = 3;
+    if (v == 4)
+        hipLaunchKernelGGL(fm_force_move_staged, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    else if (v == 3)
         hipLaunchKernelGGL(fm_force_move<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
                            cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
     else if (v == 2)

@@ -31,6 +31,9 @@ struct gell_ctx {
     int2* se_lin = nullptr;        // linear-layout copy for gell_voxel_ranges
     int64_t KS = 0;                // Morton key space, 2^(3 ceil(log2 N))
     int32_t* se_list = nullptr;    // voxels written by the last fm_update
+    int2* mblk = nullptr;          // per aligned 8^3 Morton block: first / last sorted slot (force variant 5)
+    int64_t nmblk = 0;             // blocks: max(1, KS / 512)
+    int32_t* n_out = nullptr;      // device scalar: cells outside the domain (they sort first)
     int32_t se_list_n = 0;
 
     // Diffusion, per substrate.

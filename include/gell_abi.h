@@ -169,7 +169,15 @@ int gell_sync(gell_ctx* ctx);
  *                              27-bit mask of non-empty voxels), one pair per
  *                              lane per step of the wave's loop,
  *                          3 = per x-plane of the stencil, the first two
- *                              candidates of its nine voxels loaded together.
+ *                              candidates of its nine voxels loaded together,
+ *                          4 = as 3, with the block's records read and
+ *                              written as coalesced 16-byte chunks staged in
+ *                              LDS (needs 16-byte aligned cell arrays; 3
+ *                              otherwise),
+ *                          5 = one workgroup per aligned 8x8x8 Morton block:
+ *                              the ranges of its 10^3 halo voxels and all
+ *                              their candidates loaded into LDS once, each
+ *                              cell's walk read from there.
  *                              Default 3.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical

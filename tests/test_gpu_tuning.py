@@ -28,12 +28,16 @@ def test_lod_variants_identical(N):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-def test_fm_variants_identical():
-    N, n = 64, 20000
+@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000)])
+def test_fm_variants_identical(N, n):
+    """All force-kernel variants give the same bits; an odd count exercises
+    the staged variant's 8-byte record tail, N = 100 Morton blocks that
+    stick out of the grid (block variant), and the mixed cells include dead
+    and outside ones."""
     P = params(N, capacity=n)
     cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
     outs = []
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4, 5):
         st = GpuState(P, cells)
         st.ctx.tune("fm_prefetch", v)
         for _ in range(3):
@@ -43,6 +47,8 @@ def test_fm_variants_identical():
     assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
     assert np.array_equal(outs[0].view(np.uint8), outs[2].view(np.uint8))
     assert np.array_equal(outs[0].view(np.uint8), outs[3].view(np.uint8))
+    assert np.array_equal(outs[0].view(np.uint8), outs[4].view(np.uint8))
+    assert np.array_equal(outs[0].view(np.uint8), outs[5].view(np.uint8))
 
 
 def test_bad_knob_is_an_error():
