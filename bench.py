@@ -97,11 +97,16 @@ def read_traffic(name: str):
     return None
 
 
-def gpu_leg(cfg, P, cells, steps, warmup):
+def gpu_leg(cfg, P, cells, steps, warmup, graph=False):
+    """Timed biology steps of the workload. With graph, the steps run through
+    the library's graph path (a one-replicate ensemble: each mechanics step
+    after the first of a biology step is one hipGraph launch); otherwise
+    through the per-step calls."""
     from gell_amd import runtime as rt
 
     n = cfg["cells"]
-    ctx = rt.Context(P)
+    ens = rt.Ensemble([P]) if graph else None
+    ctx = ens.members[0] if graph else rt.Context(P)
     buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
     buf.upload(cells)
     mech = rt.float_loop_count(P.dt_bio, P.dt_mech)
@@ -115,6 +120,10 @@ def gpu_leg(cfg, P, cells, steps, warmup):
     # one substrate before the next gives the same fields as the reference's
     # interleaved order while keeping one field's working set in cache.
     def bio_step(b, n):
+        if graph:
+            agents = n * mech  # counts are fixed between birth and death
+            m = ens.bio_step([buf], [n], b)[0]
+            return m, agents
         n = ctx.birth(buf, n, b)
         agents = 0
         for _ in range(mech):
@@ -159,7 +168,7 @@ def gpu_leg(cfg, P, cells, steps, warmup):
     out = dict(seconds=t1 - t0, agents=agents, n_final=n, mech=mech, diff=diff, prof=prof,
                device=rt.device_name(), tuned=tuned)
     buf.free()
-    ctx.close()
+    (ens if graph else ctx).close()
     return out
 
 
@@ -504,6 +513,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1, help="untimed biology steps")
     ap.add_argument("--config", choices=sorted(CONFIGS) + ["B", "N"], default="L")
     ap.add_argument("--weights", default=str(SURROGATE_WEIGHTS), help="surrogate parameters (config N)")
+    ap.add_argument("--graph", action="store_true", help="time through the graph path (one-replicate ensemble)")
     ap.add_argument("--sur-tol", type=float, default=1e-3, help="surrogate acceptance tolerance (config N)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
@@ -530,7 +540,7 @@ def main() -> int:
         knobs = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.knob)}
         print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup, knobs)), flush=True)
         return 0
-    g = gpu_leg(cfg, P, cells, args.steps, args.warmup)
+    g = gpu_leg(cfg, P, cells, args.steps, args.warmup, graph=cfg.get("graph", False) or args.graph)
 
     secs = g["seconds"]
     V = cfg["N"] ** 3
@@ -579,6 +589,7 @@ def main() -> int:
                          f"{g['mech']} x (FM + uptake maps + {g['diff']} LOD substeps per substrate) + death"),
             "grid": cfg["N"],
             "cells": cfg["cells"],
+            "path": "graph (one-replicate ensemble)" if (cfg.get("graph", False) or args.graph) else "per-step calls",
             "substrates": P.n_substrates,
             "mech_steps_per_step": g["mech"],
             "lod_substeps_per_mech_step": g["diff"],
