@@ -99,6 +99,37 @@ inline int32_t CellDeath_kernel(Context& c, gell_cell* d_cells, int32_t currentn
     return n;
 }
 
+// The reference's output files (FileSave.cuh:24-86), from host arrays.
+inline void io_check(int st, const char* where) {
+    if (st != GELL_OK) throw Error(st, std::string(where) + ": " + gell_io_last_error());
+}
+inline void savecsv_cell(const gell_cell* host_cells, int32_t currentnum, const std::string& f) {
+    io_check(gell_write_cells_csv(host_cells, currentnum, f.c_str()), "savecsv_cell");
+}
+inline void savecsv_mesh(const float* host_field, int32_t N, const std::string& f) {
+    io_check(gell_write_field_csv(host_field, N, f.c_str()), "savecsv_mesh");
+}
+inline void savecsv_meshslice(const float* host_field, int32_t N, const std::string& f) {
+    io_check(gell_write_slice_csv(host_field, N, f.c_str()), "savecsv_meshslice");
+}
+inline void savecsv_array(const int32_t* cnum, const double* time, int32_t len, const std::string& f) {
+    io_check(gell_write_counts_csv(cnum, time, len, f.c_str()), "savecsv_array");
+}
+inline void savecsv_gaparray(const int32_t* cnum, const double* time, int32_t n_iter, int32_t gap,
+                             const std::string& f) {
+    io_check(gell_write_gap_counts_csv(cnum, time, n_iter, gap, f.c_str()), "savecsv_gaparray");
+}
+
+// Binary checkpoint of a context's state and the cells (no counterpart in the
+// reference); a run restarted from it continues bit for bit.
+inline void checkpoint_save(Context& c, const gell_cell* d_cells, int32_t n, uint64_t bio_step,
+                            const std::string& f) {
+    io_check(gell_checkpoint_save(c.get(), d_cells, n, bio_step, f.c_str()), "checkpoint_save");
+}
+inline void checkpoint_load(Context& c, gell_cell* d_cells, int32_t* n, uint64_t* bio_step, const std::string& f) {
+    io_check(gell_checkpoint_load(c.get(), d_cells, n, bio_step, f.c_str()), "checkpoint_load");
+}
+
 // Trip count of `for (float t = 0; t < end; t += step)` (Main.cu:108,114,121).
 inline int32_t float_loop_count(float end, float step) {
     int32_t c = 0;

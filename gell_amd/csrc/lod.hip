@@ -18,7 +18,10 @@
 //   lod_yz_reg<N>  y and z sweeps: neighbouring lanes own neighbouring lines,
 //                  so each step of the recurrence is one coalesced 256-byte
 //                  wave access; the whole line stays in registers.
-//   lod_sweep_any  any N: one lane per line, Y kept in place in the field.
+//   lod_x_flat<N>  the x sweep for N = 100 or 200 (not multiples of the
+//                  32-voxel chunk): 64 lines staged whole through LDS.
+//   lod_sweep_any  any other N: one lane per line, Y kept in place in the
+//                  field.
 // Bytes per voxel-update (algorithmic): x 12 (p, p_rate in; p out), y 8, z 8.
 #include <cmath>
 
@@ -120,6 +123,59 @@ __global__ __launch_bounds__(64) void lod_x_reg(float* __restrict__ p, const flo
         }
         __syncthreads();
     }
+}
+
+// x sweep for an N that is not a multiple of the chunk (the reference's own
+// 100, say). The 64 lines of a wave are 64 N contiguous floats, so they are
+// read and written as 16-byte accesses over the whole span (a scalar tail when
+// 64 N is not a multiple of 4) through an LDS tile of odd row pitch: each
+// lane's reads of its own row are then bank-conflict free. Same arithmetic as
+// lod_x_reg.
+template <int N>
+__global__ __launch_bounds__(64) void lod_x_flat(float* __restrict__ p, const float* __restrict__ rate,
+                                                 const float* __restrict__ E, const float* __restrict__ F,
+                                                 float side, float bnd, float dt) {
+    constexpr int64_t NN = (int64_t)N * N;
+    constexpr int LD = N | 1;
+    __shared__ float tile[64 * LD];
+    const int lane = threadIdx.x;
+    const int64_t L0 = (int64_t)blockIdx.x * 64;
+    const int nl = (int)(NN - L0 < 64 ? NN - L0 : 64);
+    const int ne = nl * N;
+    float* __restrict__ pb = p + L0 * N;
+    const float* __restrict__ rb = rate + L0 * N;
+    auto put = [&](int idx, float x, float r) {
+        const int l = idx / N, t = idx - l * N;
+        const int64_t L = L0 + l;
+        const int j = (int)(L % N), k = (int)(L / N);
+        if (on_boundary(t, j, k, N)) x = bnd;
+        tile[l * LD + t] = x / (1.f - r * dt);
+    };
+    const int n4 = ne / 4;
+    for (int q = lane; q < n4; q += 64) {
+        const float4 a = reinterpret_cast<const float4*>(pb)[q];
+        const float4 r = reinterpret_cast<const float4*>(rb)[q];
+        put(4 * q, a.x, r.x);
+        put(4 * q + 1, a.y, r.y);
+        put(4 * q + 2, a.z, r.z);
+        put(4 * q + 3, a.w, r.w);
+    }
+    for (int idx = 4 * n4 + lane; idx < ne; idx += 64) put(idx, pb[idx], rb[idx]);
+    __syncthreads();
+    float v[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) v[t] = tile[lane * LD + t];
+    thomas_regs<N>(v, E, F, side, bnd);  // rows of lanes >= nl hold no line and are never stored
+#pragma unroll
+    for (int t = 0; t < N; ++t) tile[lane * LD + t] = v[t];
+    __syncthreads();
+    auto get = [&](int idx) {
+        const int l = idx / N;
+        return tile[l * LD + (idx - l * N)];
+    };
+    for (int q = lane; q < n4; q += 64)
+        reinterpret_cast<float4*>(pb)[q] = make_float4(get(4 * q), get(4 * q + 1), get(4 * q + 2), get(4 * q + 3));
+    for (int idx = 4 * n4 + lane; idx < ne; idx += 64) pb[idx] = get(idx);
 }
 
 // DIR 1: y lines (i, k), DIR 2: z lines (i, j); lane = flat line index.
@@ -547,6 +603,22 @@ hipError_t launch_reg(float* p, const float* rate, const float* E, const float* 
     return hipGetLastError();
 }
 
+// Serial line kernels for an N that is not a multiple of the chunk.
+template <int N>
+hipError_t launch_reg_flat(float* p, const float* rate, const float* E, const float* F, float side, float bnd,
+                           float dt, int spill, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    hipLaunchKernelGGL((lod_x_flat<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
+    if (spill > 0) {
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    } else {
+        hipLaunchKernelGGL((lod_yz_reg<N, 1, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    }
+    return hipGetLastError();
+}
+
 template <int N, int P>
 void launch_x_part(float* p, const float* rate, const float* tab, float side, float bnd, float dt, hipStream_t st) {
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
@@ -679,6 +751,8 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
         case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, spill, st);
         case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, spill, st);
         case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 100: return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, st);
         default: break;
     }
     const int64_t lines = (int64_t)N * N;

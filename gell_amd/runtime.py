@@ -159,6 +159,15 @@ def load_library() -> C.CDLL:
         "gell_surrogate_advance": (C.c_int, [vp, vp, i32, C.c_float, P(i32), P(C.c_float)]),
         "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
         "gell_nn_conv3_wgrad": (C.c_int, [vp, vp, vp, i32, i32, i32]),
+        "gell_io_last_error": (C.c_char_p, []),
+        "gell_sphere_init": (C.c_int, [vp, i32, i32, P(Params), C.c_float, u64]),
+        "gell_write_cells_csv": (C.c_int, [vp, i32, C.c_char_p]),
+        "gell_write_field_csv": (C.c_int, [vp, i32, C.c_char_p]),
+        "gell_write_slice_csv": (C.c_int, [vp, i32, C.c_char_p]),
+        "gell_write_counts_csv": (C.c_int, [vp, vp, i32, C.c_char_p]),
+        "gell_write_gap_counts_csv": (C.c_int, [vp, vp, i32, i32, C.c_char_p]),
+        "gell_checkpoint_save": (C.c_int, [vp, vp, i32, u64, C.c_char_p]),
+        "gell_checkpoint_load": (C.c_int, [vp, vp, P(i32), P(u64), C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -355,6 +364,22 @@ class Context:
     def birth_slots(self, n: int) -> np.ndarray:
         self.sync()
         return download_ptr(self.lib.gell_birth_slots(self.h), np.int32, n)
+
+    # -- restart ---------------------------------------------------------------
+    def checkpoint_save(self, cells: DeviceBuffer | int, n: int, bio_step: int, path: str) -> None:
+        """Write the state a step depends on: parameters, step counter,
+        cells[0, n), every substrate's field."""
+        st = self.lib.gell_checkpoint_save(self.h, _ptr(cells), n, bio_step, str(path).encode())
+        if st != 0:
+            raise GellError(f"gell_checkpoint_save: {self.lib.gell_io_last_error().decode()} ({st})")
+
+    def checkpoint_load(self, cells: DeviceBuffer | int, path: str) -> tuple[int, int]:
+        """Restore a checkpoint into this context and `cells`; returns (n, bio_step)."""
+        n, step = C.c_int32(), C.c_uint64()
+        st = self.lib.gell_checkpoint_load(self.h, _ptr(cells), C.byref(n), C.byref(step), str(path).encode())
+        if st != 0:
+            raise GellError(f"gell_checkpoint_load: {self.lib.gell_io_last_error().decode()} ({st})")
+        return n.value, step.value
 
     # -- kernel variants -------------------------------------------------------
     KNOBS = {"lod_yz_lds": 0, "fm_prefetch": 1, "lod_parts": 2, "lod_x_parts": 3, "lod_fused": 4,
@@ -581,3 +606,63 @@ def device_name() -> str:
     if load_library().gell_device_name(buf, 256) != 0:
         raise GellError("no device")
     return buf.value.decode()
+
+
+# -- initial state and output files (host arrays) -------------------------------
+def sphere_init(params: Params, n: int, capacity: int | None = None, cell_radius: float = 8.4,
+                seed: int = 12345) -> np.ndarray:
+    """Cell_sphere_Initialization (Initialization.cuh:15-43): `capacity`
+    records, the first n quiescent cells uniformly in the ball of radius
+    cell_radius n^(1/3) / 1.612 * 1.87 at the domain centre (SplitMix64 draws)."""
+    cap = params.capacity if capacity is None else capacity
+    cells = np.zeros(cap, CELL_DTYPE)
+    st = load_library().gell_sphere_init(cells.ctypes.data, cap, n, C.byref(params), cell_radius, seed)
+    _io_check(st, "gell_sphere_init")
+    return cells
+
+
+
+def _io_check(st: int, what: str) -> None:
+    if st != 0:
+        raise GellError(f"{what}: {load_library().gell_io_last_error().decode()} ({st})")
+
+
+def savecsv_cell(cells: np.ndarray, n: int, path: str) -> None:
+    """savecsv_cell (FileSave.cuh:24-33) of records cells[0, n) (CELL_DTYPE)."""
+    c = np.ascontiguousarray(cells[:n], dtype=CELL_DTYPE)
+    _io_check(load_library().gell_write_cells_csv(c.ctypes.data if n else None, n, str(path).encode()),
+              "gell_write_cells_csv")
+
+
+def savecsv_mesh(field: np.ndarray, N: int, path: str) -> None:
+    """savecsv_mesh (FileSave.cuh:35-49) of an N^3 field (x fastest)."""
+    f = np.ascontiguousarray(field, dtype=np.float32).ravel()
+    assert f.size == N ** 3
+    _io_check(load_library().gell_write_field_csv(f.ctypes.data, N, str(path).encode()), "gell_write_field_csv")
+
+
+def savecsv_meshslice(field: np.ndarray, N: int, path: str) -> None:
+    """savecsv_meshslice (FileSave.cuh:50-64) of the plane z = N / 2."""
+    f = np.ascontiguousarray(field, dtype=np.float32).ravel()
+    assert f.size == N ** 3
+    _io_check(load_library().gell_write_slice_csv(f.ctypes.data, N, str(path).encode()), "gell_write_slice_csv")
+
+
+def savecsv_array(counts: np.ndarray, times: np.ndarray, path: str) -> None:
+    """savecsv_array (FileSave.cuh:66-75): one row per entry."""
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    assert c.size == t.size
+    _io_check(load_library().gell_write_counts_csv(c.ctypes.data, t.ctypes.data, c.size, str(path).encode()),
+              "gell_write_counts_csv")
+
+
+def savecsv_gaparray(counts: np.ndarray, gap_times: np.ndarray, n_iter: int, gap: int, path: str) -> None:
+    """savecsv_gaparray (FileSave.cuh:77-86): rows i < n_iter // gap of
+    counts[(i + 1) gap - 1] and gap_times[i]."""
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    t = np.ascontiguousarray(gap_times, dtype=np.float64)
+    rows = n_iter // gap
+    assert c.size >= rows * gap and t.size >= rows
+    _io_check(load_library().gell_write_gap_counts_csv(c.ctypes.data, t.ctypes.data, n_iter, gap,
+                                                        str(path).encode()), "gell_write_gap_counts_csv")

@@ -256,6 +256,53 @@ int gell_device_name(char* buf, int len);
 int32_t gell_float_loop_count(float end, float step);
 
 /* ---------------------------------------------------------------------------
+ * Output files and restart. The CSV writers take HOST arrays (copy the cells
+ * or a field back first, as the reference does before saving) and print with
+ * the reference's format strings. Each returns GELL_OK or GELL_E_ARG (bad
+ * arguments, file not writable); gell_io_last_error() says which.
+ */
+const char* gell_io_last_error(void);
+/* Cell_sphere_Initialization (Initialization.cuh:15-43), host-side: every
+ * record of cells[0, capacity) set to Cell's defaults (Cells.cuh:10-33),
+ * then n quiescent cells of radius `cell_radius` (V 2494, Vf 1871, Vns 135,
+ * Vcs 489) placed uniformly in the ball of radius
+ * cell_radius n^(1/3) / 1.612 * 1.87 around the domain centre, with
+ * mesh_idx set. The draws come from SplitMix64 on `seed` (the reference
+ * uses the C library's rand(), whose sequence is platform-specific). */
+int gell_sphere_init(gell_cell* cells, int32_t capacity, int32_t n, const gell_params* params, float cell_radius,
+                     uint64_t seed);
+/* savecsv_cell (FileSave.cuh:24-33): "Index,Sign,Phase,x,y,z,r,V,Vns,Vcs",
+ * one row per record of cells[0, n). */
+int gell_write_cells_csv(const gell_cell* cells, int32_t n, const char* path);
+/* savecsv_mesh (FileSave.cuh:35-49): "i,j,k,Concentration", N^3 rows, x
+ * fastest. */
+int gell_write_field_csv(const float* field, int32_t N, const char* path);
+/* savecsv_meshslice (FileSave.cuh:50-64) of the plane k = N / 2 that
+ * Slice_to_CPU copies (LODkernel.cuh:174-181, 301-306): "x,y,Concentration",
+ * N^2 rows. `field` is the whole N^3 field. */
+int gell_write_slice_csv(const float* field, int32_t N, const char* path);
+/* savecsv_array (FileSave.cuh:66-75): "Index,Cell Num,Time", rows
+ * i = 0 .. len-1 (the reference writes Max_Simulation_Iter + 1 of them). */
+int gell_write_counts_csv(const int32_t* counts, const double* times, int32_t len, const char* path);
+/* savecsv_gaparray (FileSave.cuh:77-86): rows i < n_iter / gap with
+ * counts[(i + 1) gap - 1] and gap_times[i], the reference's indexing (it
+ * passes Save_data_gap = 360, in minutes, as `gap`). */
+int gell_write_gap_counts_csv(const int32_t* counts, const double* gap_times, int32_t n_iter, int32_t gap,
+                              const char* path);
+
+/* Binary checkpoint (no counterpart in the reference): the context's
+ * parameters, the biology-step counter, d_cells[0, n) and every substrate's
+ * field. That is the whole state a step depends on (the cell list and the
+ * uptake maps are rebuilt by every mechanics step; the birth draws are keyed
+ * on the step counter), so a run restarted from a checkpoint continues bit
+ * for bit as the uninterrupted run. Load needs a context with the same N,
+ * dx and substrate count and room for the saved records; it writes the
+ * records to d_cells and the count and step to *n and *bio_step.
+ * Both synchronise; a failed load changes nothing on the device. */
+int gell_checkpoint_save(gell_ctx* ctx, const gell_cell* d_cells, int32_t n, uint64_t bio_step, const char* path);
+int gell_checkpoint_load(gell_ctx* ctx, gell_cell* d_cells, int32_t* n, uint64_t* bio_step, const char* path);
+
+/* ---------------------------------------------------------------------------
  * Ensembles: many independent replicates (each its own N, dx, capacity,
  * seed and substrates; shared dt_bio / dt_mech / dt_diff) advanced together.
  * The reference runs one simulation per process (Main.cu:61-127); this is

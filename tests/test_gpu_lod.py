@@ -32,7 +32,7 @@ def _gpu_substeps(P, p, rate, steps, s=0, exact=True):
     return out
 
 
-@pytest.mark.parametrize("N", [16, 32, 64, 96, 100, 128])
+@pytest.mark.parametrize("N", [16, 32, 50, 64, 96, 100, 128, 200])
 def test_lod_substeps_match_oracle(N, oracle):
     log(f"[lod] N={N}")
     P = params(N)
@@ -45,6 +45,27 @@ def test_lod_substeps_match_oracle(N, oracle):
     log(f"[lod] N={N} relL2={e:.3e} max_ulp={u}")
     assert e <= 1e-5
     assert u == 0
+
+
+@pytest.mark.parametrize("spill", [0, 1])
+def test_lod_100_register_and_lds_variants(spill, oracle):
+    """N = 100 (the reference's Voxel_num) runs the LDS-staged x sweep and
+    the register y/z sweeps, with or without the LDS spill: bit-exact."""
+    N = 100
+    P = params(N, dx=25.0)
+    p, rate = _field_and_rate(N, 7 + spill)
+    st = GpuState(P, np.zeros(0, rt.CELL_DTYPE), exact=True)
+    st.ctx.tune("lod_yz_lds", spill)
+    st.ctx.set_field(0, p)
+    rt.upload_ptr(st.ctx.lib.gell_uptake_field(st.ctx.h, 0), rate)
+    for _ in range(4):
+        st.ctx.lod_substep(0)
+    got = st.ctx.field(0)
+    st.close()
+    want = p.copy()
+    for _ in range(4):
+        oracle.lod_substep(P, 0, want, rate)
+    assert ulp_diff(got, want) == 0
 
 
 def test_lod_256_one_substep(oracle):
