@@ -20,6 +20,7 @@
 //                  wave access; the whole line stays in registers.
 //   lod_x_flat<N>  the x sweep for N = 100 or 200 (not multiples of the
 //                  32-voxel chunk): 64 lines staged whole through LDS.
+//   lod_xy_plane<N> x and y sweeps of one z plane in one workgroup (N = 100).
 //   lod_sweep_any  any other N: one lane per line, Y kept in place in the
 //                  field.
 // Bytes per voxel-update (algorithmic): x 12 (p, p_rate in; p out), y 8, z 8.
@@ -126,41 +127,57 @@ __global__ __launch_bounds__(64) void lod_x_reg(float* __restrict__ p, const flo
 }
 
 // x sweep for an N that is not a multiple of the chunk (the reference's own
-// 100, say). The 64 lines of a wave are 64 N contiguous floats, so they are
-// read and written as 16-byte accesses over the whole span (a scalar tail when
-// 64 N is not a multiple of 4) through an LDS tile of odd row pitch: each
-// lane's reads of its own row are then bank-conflict free. Same arithmetic as
+// 100, say; N % 4 == 0). The 64 lines of a wave are 64 N contiguous floats,
+// read and written as coalesced 16-byte accesses (lane-consecutive float4s)
+// through an LDS tile of odd row pitch, so that each lane's reads of its own
+// row are bank-conflict free. The loads are issued in unrolled batches of kB
+// float4 per array, so a wave has them all in flight at once: with one wave
+// per CU at these grid sizes, latency is the cost. Same arithmetic as
 // lod_x_reg.
 template <int N>
 __global__ __launch_bounds__(64) void lod_x_flat(float* __restrict__ p, const float* __restrict__ rate,
                                                  const float* __restrict__ E, const float* __restrict__ F,
                                                  float side, float bnd, float dt) {
-    constexpr int64_t NN = (int64_t)N * N;
+    static_assert(N % 4 == 0 && N * N < (1 << 30), "line span must be whole float4s");
+    constexpr int NN = N * N;
+    constexpr int Q = N / 4;  // float4 per line = float4 per lane per array
+    constexpr int kB = 13;
     constexpr int LD = N | 1;
     __shared__ float tile[64 * LD];
     const int lane = threadIdx.x;
-    const int64_t L0 = (int64_t)blockIdx.x * 64;
-    const int nl = (int)(NN - L0 < 64 ? NN - L0 : 64);
-    const int ne = nl * N;
-    float* __restrict__ pb = p + L0 * N;
-    const float* __restrict__ rb = rate + L0 * N;
-    auto put = [&](int idx, float x, float r) {
-        const int l = idx / N, t = idx - l * N;
-        const int64_t L = L0 + l;
-        const int j = (int)(L % N), k = (int)(L / N);
-        if (on_boundary(t, j, k, N)) x = bnd;
-        tile[l * LD + t] = x / (1.f - r * dt);
-    };
-    const int n4 = ne / 4;
-    for (int q = lane; q < n4; q += 64) {
-        const float4 a = reinterpret_cast<const float4*>(pb)[q];
-        const float4 r = reinterpret_cast<const float4*>(rb)[q];
-        put(4 * q, a.x, r.x);
-        put(4 * q + 1, a.y, r.y);
-        put(4 * q + 2, a.z, r.z);
-        put(4 * q + 3, a.w, r.w);
+    const int L0 = blockIdx.x * 64;
+    const int nl = NN - L0 < 64 ? NN - L0 : 64;
+    const int n4 = nl * Q;
+    const float4* __restrict__ pb = reinterpret_cast<const float4*>(p + (int64_t)L0 * N);
+    const float4* __restrict__ rb = reinterpret_cast<const float4*>(rate + (int64_t)L0 * N);
+#pragma unroll
+    for (int b0 = 0; b0 < Q; b0 += kB) {
+        float4 a[kB], r[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * 64 + lane;
+            if (b0 + u < Q && q < n4) {
+                a[u] = pb[q];
+                r[u] = rb[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * 64 + lane;
+            if (b0 + u < Q && q < n4) {
+                const int l = q / Q, t0 = 4 * (q - l * Q);
+                const int L = L0 + l, j = L % N, k = L / N;
+                const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+                const float rv[4] = {r[u].x, r[u].y, r[u].z, r[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float x = av[e];
+                    if (on_boundary(t0 + e, j, k, N)) x = bnd;
+                    tile[l * LD + t0 + e] = x / (1.f - rv[e] * dt);
+                }
+            }
+        }
     }
-    for (int idx = 4 * n4 + lane; idx < ne; idx += 64) put(idx, pb[idx], rb[idx]);
     __syncthreads();
     float v[N];
 #pragma unroll
@@ -169,13 +186,109 @@ __global__ __launch_bounds__(64) void lod_x_flat(float* __restrict__ p, const fl
 #pragma unroll
     for (int t = 0; t < N; ++t) tile[lane * LD + t] = v[t];
     __syncthreads();
-    auto get = [&](int idx) {
-        const int l = idx / N;
-        return tile[l * LD + (idx - l * N)];
-    };
-    for (int q = lane; q < n4; q += 64)
-        reinterpret_cast<float4*>(pb)[q] = make_float4(get(4 * q), get(4 * q + 1), get(4 * q + 2), get(4 * q + 3));
-    for (int idx = 4 * n4 + lane; idx < ne; idx += 64) pb[idx] = get(idx);
+    float4* __restrict__ po = reinterpret_cast<float4*>(p + (int64_t)L0 * N);
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+        const int q = u * 64 + lane;
+        if (q < n4) {
+            const int l = q / Q, t0 = 4 * (q - l * Q);
+            const float* src = &tile[l * LD + t0];
+            po[q] = make_float4(src[0], src[1], src[2], src[3]);
+        }
+    }
+}
+
+// x and y sweeps of one z plane in one workgroup, for N whose plane fits in
+// LDS (N <= 128; used at N = 100): the plane is read once with coalesced
+// float4 accesses (boundary and implicit uptake applied on the way in, as in
+// lod_x_reg), lane j solves x line j out of the odd-pitch tile, then lane i
+// solves y line i (lane-consecutive addresses, conflict free), and the plane
+// is written back once. The y sweep keeps lod_yz_reg's operation order, so
+// the result equals the separate x and y kernels bit for bit, with half the
+// launches and one pass over the field instead of two.
+template <int N>
+__global__ __launch_bounds__(128) void lod_xy_plane(float* __restrict__ p, const float* __restrict__ rate,
+                                                    const float* __restrict__ E, const float* __restrict__ F,
+                                                    float side, float bnd, float dt) {
+    static_assert(N % 4 == 0 && N <= 128, "plane must fit in LDS as whole float4 rows");
+    constexpr int Q = N / 4;
+    constexpr int NQ = N * Q;                // float4 per plane
+    constexpr int PER = (NQ + 127) / 128;    // float4 per thread
+    constexpr int kB = 10;
+    constexpr int LD = N | 1;
+    __shared__ float tile[N * LD];
+    const int tid = threadIdx.x;
+    const int k = blockIdx.x;
+    float* __restrict__ plane = p + (int64_t)k * N * N;
+    const float4* __restrict__ pb = reinterpret_cast<const float4*>(plane);
+    const float4* __restrict__ rb = reinterpret_cast<const float4*>(rate + (int64_t)k * N * N);
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += kB) {
+        float4 a[kB], r[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * 128 + tid;
+            if (b0 + u < PER && q < NQ) {
+                a[u] = pb[q];
+                r[u] = rb[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * 128 + tid;
+            if (b0 + u < PER && q < NQ) {
+                const int j = q / Q, i0 = 4 * (q - j * Q);
+                const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+                const float rv[4] = {r[u].x, r[u].y, r[u].z, r[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float x = av[e];
+                    if (on_boundary(i0 + e, j, k, N)) x = bnd;
+                    tile[j * LD + i0 + e] = x / (1.f - rv[e] * dt);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < N) {  // x line j = tid
+        float v[N];
+#pragma unroll
+        for (int t = 0; t < N; ++t) v[t] = tile[tid * LD + t];
+        thomas_regs<N>(v, E, F, side, bnd);
+#pragma unroll
+        for (int t = 0; t < N; ++t) tile[tid * LD + t] = v[t];
+    }
+    __syncthreads();
+    if (tid < N) {  // y line i = tid, as lod_yz_reg<N, 1, 0>
+        float v[N];
+#pragma unroll
+        for (int t = 1; t < N - 1; ++t) v[t] = tile[t * LD + tid];
+        float y = bnd;
+#pragma unroll
+        for (int t = 1; t < N - 1; ++t) {
+            y = v[t] - E[t] * y;
+            v[t] = y;
+        }
+        tile[(N - 1) * LD + tid] = bnd;
+        float nxt = bnd;
+#pragma unroll
+        for (int t = N - 2; t > 0; --t) {
+            nxt = (v[t] - side * nxt) / F[t];
+            tile[t * LD + tid] = nxt;
+        }
+        tile[tid] = bnd;
+    }
+    __syncthreads();
+    float4* __restrict__ po = reinterpret_cast<float4*>(plane);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int q = u * 128 + tid;
+        if (q < NQ) {
+            const int j = q / Q, i0 = 4 * (q - j * Q);
+            const float* src = &tile[j * LD + i0];
+            po[q] = make_float4(src[0], src[1], src[2], src[3]);
+        }
+    }
 }
 
 // DIR 1: y lines (i, k), DIR 2: z lines (i, j); lane = flat line index.
@@ -603,11 +716,22 @@ hipError_t launch_reg(float* p, const float* rate, const float* E, const float* 
     return hipGetLastError();
 }
 
-// Serial line kernels for an N that is not a multiple of the chunk.
+// Serial line kernels for an N that is not a multiple of the chunk; with
+// `plane` (N <= 128) the x and y sweeps run as one kernel per z plane.
 template <int N>
 hipError_t launch_reg_flat(float* p, const float* rate, const float* E, const float* F, float side, float bnd,
-                           float dt, int spill, hipStream_t st) {
+                           float dt, int spill, bool plane, hipStream_t st) {
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    if constexpr (N <= 128) {
+        if (plane) {
+            hipLaunchKernelGGL((lod_xy_plane<N>), dim3(N), dim3(128), 0, st, p, rate, E, F, side, bnd, dt);
+            if (spill > 0)
+                hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+            else
+                hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((lod_x_flat<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
     if (spill > 0) {
         hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
@@ -751,8 +875,8 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
         case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, spill, st);
         case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, spill, st);
         case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, spill, st);
-        case 100: return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, st);
-        case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 100: return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, ctx->tune[GELL_KNOB_LOD_FUSED], st);
+        case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, false, st);
         default: break;
     }
     const int64_t lines = (int64_t)N * N;
@@ -889,6 +1013,8 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
     if (part_ok && ctx->coup[0])
         for (int yi = 1; yi < 5; ++yi)
             if (N % (cand[yi] * 32) == 0) cands.push_back({N / 64, cand[yi], 0, 1});
+    if (N == 100)  // x + y per z plane (serial lines, so also bit-exact)
+        for (int sp = 0; sp < 2; ++sp) cands.push_back({1, 1, sp, 1});
     const int knobs[4] = {GELL_KNOB_LOD_X_PARTS, GELL_KNOB_LOD_PARTS, GELL_KNOB_LOD_YZ_LDS, GELL_KNOB_LOD_FUSED};
     int saved[4], best[4];
     for (int q = 0; q < 4; ++q) saved[q] = best[q] = ctx->tune[knobs[q]];

@@ -198,7 +198,10 @@ int gell_sync(gell_ctx* ctx);
  *                              traffic per voxel instead of 28, x and y in
  *                              N / 64 blocks, z in LOD_PARTS blocks; results
  *                              equal the unfused kernels with those block
- *                              counts bit for bit. Default 1.
+ *                              counts bit for bit. At N = 100 (serial
+ *                              lines) 1 = the x and y sweeps of each z plane
+ *                              in one workgroup, bit-identical to the
+ *                              separate sweeps. Default 1.
  *   GELL_KNOB_SUBSTRATE_STREAMS 1 = each substrate's LOD substeps run on a
  *                              stream of its own, overlapping the other
  *                              substrates' substeps and the next mechanics

@@ -47,15 +47,18 @@ def test_lod_substeps_match_oracle(N, oracle):
     assert u == 0
 
 
+@pytest.mark.parametrize("plane", [0, 1])
 @pytest.mark.parametrize("spill", [0, 1])
-def test_lod_100_register_and_lds_variants(spill, oracle):
-    """N = 100 (the reference's Voxel_num) runs the LDS-staged x sweep and
-    the register y/z sweeps, with or without the LDS spill: bit-exact."""
+def test_lod_100_register_and_lds_variants(spill, plane, oracle):
+    """N = 100 (the reference's Voxel_num): the LDS-staged x sweep and the
+    register y/z sweeps, or the x + y sweeps per z plane in one workgroup,
+    with or without the LDS spill of the z sweep: all bit-exact."""
     N = 100
     P = params(N, dx=25.0)
-    p, rate = _field_and_rate(N, 7 + spill)
+    p, rate = _field_and_rate(N, 7 + spill + 2 * plane)
     st = GpuState(P, np.zeros(0, rt.CELL_DTYPE), exact=True)
     st.ctx.tune("lod_yz_lds", spill)
+    st.ctx.tune("lod_fused", plane)
     st.ctx.set_field(0, p)
     rt.upload_ptr(st.ctx.lib.gell_uptake_field(st.ctx.h, 0), rate)
     for _ in range(4):
