@@ -327,6 +327,17 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
 // ---------------------------------------------------------------------------
 // One candidate pair of forcekernel (FMkernel.cuh:86-109): returns false
 // when the pair is skipped; thres carries over between calls.
+// One 16-byte neighbour entry. The raw buffer load cannot be split: a plain
+// float4 load followed by the dead-tag test on .w was compiled into a 4-byte
+// load, a wait, and a conditional 12-byte load -- two memory round trips per
+// candidate.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t nb_rsrc(const float4* nb) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(nb), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ float4 load_nb(__amdgpu_buffer_rsrc_t rs, int32_t j) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, j * 16, 0, 0));
+}
+
 __device__ __forceinline__ void force_pair(float4 o, float p1x, float p1y, float p1z, float r1, float& thres,
                                            float& fx, float& fy, float& fz) {
     if (__float_as_uint(o.w) == kDeadTag) return;
@@ -358,6 +369,7 @@ __device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb
                                                    float& fx, float& fy, float& fz) {
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
     float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     int2 rr[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
@@ -370,8 +382,8 @@ __device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int2 r = rr[g * 9 + t];
-            c0[t] = r.y >= 0 ? nb[r.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-            c1[t] = r.y > r.x ? nb[r.x + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c0[t] = r.y >= 0 ? load_nb(rs, r.x) : make_float4(0.f, 0.f, 0.f, 0.f);
+            c1[t] = r.y > r.x ? load_nb(rs, r.x + 1) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
@@ -379,20 +391,30 @@ __device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb
             if (r.y < 0) continue;
             force_pair(c0[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
             if (r.y > r.x) force_pair(c1[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
-            for (int j = r.x + 2; j <= r.y; ++j) force_pair(nb[j], p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            if (r.y > r.x + 1) {
+                float4 o = load_nb(rs, r.x + 2);
+                for (int j = r.x + 2; j <= r.y; ++j) {
+                    const float4 nxt = load_nb(rs, j < r.y ? j + 1 : j);  // one candidate ahead
+                    force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+                    o = nxt;
+                }
+            }
         }
     }
 }
 
-// Force on live cell i (forcekernel, FMkernel.cuh:56-118). With PREFETCH the
-// 27 stencil ranges are loaded before the pair loop, so their latencies
-// overlap; the visiting order (x outer, z inner, ascending slot) is the same.
+// Force on live cell i (forcekernel, FMkernel.cuh:56-118). Candidates are
+// loaded one ahead of their evaluation. With PREFETCH the 27 stencil ranges
+// are loaded before the pair loop, so their latencies overlap, and the load
+// ahead runs on across voxel boundaries; the visiting order (x outer, z
+// inner, ascending slot) is the same either way.
 template <bool PREFETCH>
 __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const int2* __restrict__ se, int32_t N,
                                            float dx, float p1x, float p1y, float p1z, float r1, float& fx, float& fy,
                                            float& fz) {
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
     float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     int2 rr[27];
     if (PREFETCH) {
 #pragma unroll
@@ -400,38 +422,41 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
             const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
             rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
         }
+        // nf[q]: first candidate of the next non-empty voxel after q (-1: none), so
+        // the load one candidate ahead runs on across voxel boundaries
+        int nf[27], next = -1;
+#pragma unroll
+        for (int q = 26; q >= 0; --q) {
+            nf[q] = next;
+            if (rr[q].y >= 0) next = rr[q].x;
+        }
+        if (next < 0) return;
+        float4 o = load_nb(rs, next);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int2 r = rr[q];
+            if (r.y < 0) continue;
+            for (int j = r.x; j <= r.y; ++j) {
+                const int ni = j < r.y ? j + 1 : (nf[q] >= 0 ? nf[q] : j);
+                const float4 nxt = load_nb(rs, ni);
+                force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+                o = nxt;
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
-        int2 r;
-        if (PREFETCH) {
-            r = rr[q];
-        } else {
-            if (!inside(xi, yi, zi, N)) continue;
-            r = se[morton3(xi, yi, zi)];
-        }
+        if (!inside(xi, yi, zi, N)) continue;
+        const int2 r = se[morton3(xi, yi, zi)];
         if (r.y < 0) continue;
+        // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
+        float4 o = load_nb(rs, r.x);
         for (int j = r.x; j <= r.y; ++j) {
-            const float4 o = nb[j];
-            if (__float_as_uint(o.w) == kDeadTag) continue;
-            // FMkernel.cuh:89-109
-            const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
-            const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
-            const float distance = sqrtf((float)((double)ss + 0.0001));
-            if (distance >= thres) continue;
-            const float rsum = o.w + r1;
-            const float R = rsum > 0.01f ? rsum : 0.01f;
-            thres = (float)((double)R * 1.25);
-            if (distance >= thres) continue;
-            const float qr = 1.f - distance / R;
-            const float qa = 1.f - distance / thres;
-            const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
-            const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
-            const float cc = coefr - coefa;
-            fx += ddx * cc;
-            fy += ddy * cc;
-            fz += ddz * cc;
+            const float4 nxt = load_nb(rs, j < r.y ? j + 1 : j);
+            force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            o = nxt;
         }
     }
 }
@@ -459,6 +484,7 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
         for (int q = 0; q < 27; ++q) mask |= (rr[q].y >= 0 ? 1u : 0u) << q;
     }
     float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     int j = 0, je = -1;
     while (true) {
         if (j > je && mask != 0) {
@@ -472,7 +498,7 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
         const bool active = j <= je;
         if (!__any(active)) break;
         if (!active) continue;
-        const float4 o = nb[j];
+        const float4 o = load_nb(rs, j);
         ++j;
         if (__float_as_uint(o.w) == kDeadTag) continue;
         // FMkernel.cuh:89-109

@@ -14,7 +14,7 @@
 //     one line per biology step: count, wall ms, agent-steps/s.
 //
 //   gell_sim hds [--out DIR] [--steps K] [--seed S] [--save-cells] [--save-mesh]
-//                [--checkpoint-every M] [--restart FILE] [--exact]
+//                [--checkpoint-every M] [--restart FILE] [--exact] [--fm V]
 //     The reference's own example (Definitions.h as shipped, Main.cu): a
 //     100^3 grid of 25 um voxels, 2347 cells of radius 8.4 um seeded by
 //     Cell_sphere_Initialization, 19 days of biology steps (4560), capacity
@@ -25,7 +25,8 @@
 //     Gell_Simuend.csv, with --save-mesh Mesh_Simuend.csv. Every M steps the
 //     state goes to DIR/checkpoint.bin; --restart continues from such a file.
 //     --exact restricts the autotuner to kernel variants that are bit-identical
-//     to the serial sweeps. Prints one line per 360 min of biology.
+//     to the serial sweeps; --fm V sets the force-kernel variant
+//     (GELL_KNOB_FM_PREFETCH). Prints one line per 360 min of biology.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -132,6 +133,7 @@ struct HdsOptions {
     uint64_t seed = 12345;
     bool save_cells = false, save_mesh = false, exact = false;
     int32_t checkpoint_every = 0;
+    int32_t fm_variant = -1;
     std::string restart;
 };
 
@@ -151,6 +153,7 @@ bool parse_hds(int argc, char** argv, HdsOptions& o) {
         else if (a == "--save-cells") o.save_cells = true;
         else if (a == "--save-mesh") o.save_mesh = true;
         else if (a == "--exact") o.exact = true;
+        else if (a == "--fm") o.fm_variant = std::atoi(next("--fm"));
         else if (a == "--checkpoint-every") o.checkpoint_every = std::atoi(next("--checkpoint-every"));
         else if (a == "--restart") o.restart = next("--restart");
         else {
@@ -180,6 +183,8 @@ int run_hds(int argc, char** argv) {
 
     gell::Context ctx(P);
     ctx.autotune(o.exact);
+    if (o.fm_variant >= 0)
+        gell::check(ctx.get(), gell_tune(ctx.get(), GELL_KNOB_FM_PREFETCH, o.fm_variant), "gell_tune");
     gell::MechanicsMesh MecMesh(ctx);
     gell::Mesh O2Mesh(ctx, 0);
 

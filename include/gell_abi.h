@@ -164,7 +164,11 @@ int gell_sync(gell_ctx* ctx);
  *                          1 = first 64 forward values per line in LDS
  *                              (two waves per SIMD), for N > 64
  *   GELL_KNOB_FM_PREFETCH  0 = one stencil-range load per voxel in turn,
- *                          1 = all 27 stencil ranges loaded up front,
+ *                              each candidate loaded one ahead of its
+ *                              evaluation,
+ *                          1 = all 27 stencil ranges loaded up front, the
+ *                              load one candidate ahead running on across
+ *                              voxel boundaries,
  *                          2 = each lane walks its own candidate list (a
  *                              27-bit mask of non-empty voxels), one pair per
  *                              lane per step of the wave's loop,
@@ -178,7 +182,7 @@ int gell_sync(gell_ctx* ctx);
  *                              the ranges of its 10^3 halo voxels and all
  *                              their candidates loaded into LDS once, each
  *                              cell's walk read from there.
- *                              Default 3.
+ *                              Default 1.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4, 8 or 16 = each line split into

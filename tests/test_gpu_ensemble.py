@@ -97,7 +97,7 @@ def test_ensemble_graph_follows_knob_changes(solo_runs):
     buf = rt.DeviceBuffer(P.capacity * 72)
     buf.upload(c)
     ens.mech_steps([buf], [SPECS[2][1]], 3)
-    ens.members[0].tune("fm_prefetch", 1)
+    ens.members[0].tune("fm_prefetch", 3)
     ens.mech_steps([buf], [SPECS[2][1]], 3)
     ens.sync()
     assert ens.stats() == {"graph_captures": 2, "graph_launches": 4}

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from gell_amd import runtime as rt
-from helpers import GpuState, mixed_cells, params
+from helpers import GpuState, compare_cells, mixed_cells, params
 
 pytestmark = pytest.mark.gpu
 
@@ -36,25 +36,45 @@ def test_fm_variants_identical(N, n):
     and outside ones."""
     P = params(N, capacity=n)
     cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
+    _all_fm_variants_equal(P, cells, n)
+
+
+def _all_fm_variants_equal(P, cells, n, steps=3):
     outs = []
     for v in (0, 1, 2, 3, 4, 5):
         st = GpuState(P, cells)
         st.ctx.tune("fm_prefetch", v)
-        for _ in range(3):
+        for _ in range(steps):
             st.ctx.fm_update(st.buf, n)
         outs.append(st.cells())
         st.close()
-    assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
-    assert np.array_equal(outs[0].view(np.uint8), outs[2].view(np.uint8))
-    assert np.array_equal(outs[0].view(np.uint8), outs[3].view(np.uint8))
-    assert np.array_equal(outs[0].view(np.uint8), outs[4].view(np.uint8))
-    assert np.array_equal(outs[0].view(np.uint8), outs[5].view(np.uint8))
+    for v in range(1, len(outs)):
+        assert np.array_equal(outs[0].view(np.uint8), outs[v].view(np.uint8)), f"variant {v}"
+
+
+@pytest.mark.parametrize("n", [12000, 40000])
+def test_fm_variants_identical_dense_hds(n, oracle):
+    """The reference's HDS tissue (radius 8.4 um cells, ~4 per 25 um voxel):
+    every variant gives the same bits, equal to the oracle's."""
+    N = 60
+    P = params(N, dx=25.0, capacity=n)
+    cells = rt.sphere_init(P, n, n, 8.4, seed=n)
+    _all_fm_variants_equal(P, cells, n)
+    st = GpuState(P, cells)
+    st.ctx.fm_update(st.buf, n)
+    got = st.cells()
+    st.close()
+    want = cells.copy()
+    oracle.fm_update(P, want, n)
+    compare_cells(got, want, n)
 
 
 def test_bad_knob_is_an_error():
     st = GpuState(params(16), np.zeros(0, rt.CELL_DTYPE))
     with pytest.raises(rt.GellError):
         st.ctx.tune("lod_yz_lds", 7)
+    with pytest.raises(rt.GellError):
+        st.ctx.tune("fm_prefetch", 6)
     with pytest.raises(KeyError):
         st.ctx.tune("nope", 1)
     st.close()
