@@ -415,33 +415,41 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
     float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
-    int2 rr[27];
     if (PREFETCH) {
+        // last[q]: last candidate of stencil voxel q (-1: empty); nf[q]: first
+        // candidate of the next non-empty voxel after q (-1: none). A voxel's
+        // first candidate is where the previous one's load ahead pointed, so
+        // the walk needs no other range data and the load one candidate ahead
+        // runs on across voxel boundaries.
+        int last[27], nf[27], next = -1;
+        {
+            int2 rr[27];
 #pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
-            rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
-        }
-        // nf[q]: first candidate of the next non-empty voxel after q (-1: none), so
-        // the load one candidate ahead runs on across voxel boundaries
-        int nf[27], next = -1;
+            for (int q = 0; q < 27; ++q) {
+                const int xi = px - 1 + q / 9, yi = py - 1 + (q / 3) % 3, zi = pz - 1 + q % 3;
+                rr[q] = inside(xi, yi, zi, N) ? se[morton3(xi, yi, zi)] : make_int2(0, -1);
+            }
 #pragma unroll
-        for (int q = 26; q >= 0; --q) {
-            nf[q] = next;
-            if (rr[q].y >= 0) next = rr[q].x;
+            for (int q = 26; q >= 0; --q) {
+                nf[q] = next;
+                last[q] = rr[q].y;
+                if (rr[q].y >= 0) next = rr[q].x;
+            }
         }
         if (next < 0) return;
-        float4 o = load_nb(rs, next);
+        int j = next;
+        float4 o = load_nb(rs, j);
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
-            const int2 r = rr[q];
-            if (r.y < 0) continue;
-            for (int j = r.x; j <= r.y; ++j) {
-                const int ni = j < r.y ? j + 1 : (nf[q] >= 0 ? nf[q] : j);
+            const int e = last[q];
+            if (e < 0) continue;
+            for (; j <= e; ++j) {
+                const int ni = j < e ? j + 1 : (nf[q] >= 0 ? nf[q] : j);
                 const float4 nxt = load_nb(rs, ni);
                 force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
                 o = nxt;
             }
+            j = nf[q];
         }
         return;
     }
@@ -560,21 +568,23 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
                                                         float dt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in_range = i < n;
-    CellRegs c;
-    if (in_range) c = load_cell(sorted, i);
-    const bool live = in_range && csign(c);
+    // The cell's own (x, y, z, r) and liveness come from its 16-byte entry of
+    // the compact array (same values as its record); the 72-byte record is
+    // read after the pair loop, so it holds no registers during the walk.
+    const float4 me = in_range ? load_nb(nb_rsrc(nb), (int32_t)i) : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
+    const bool live = __float_as_uint(me.w) != kDeadTag;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     if (VARIANT == 2) {
         // every lane of the wave takes part in the candidate loop's vote
-        const float p1x = live ? cf(c, W_PX) : 0.f, p1y = live ? cf(c, W_PY) : 0.f,
-                    p1z = live ? cf(c, W_PZ) : 0.f;
-        cell_force_flat(nb, se, N, dx, live, p1x, p1y, p1z, live ? cf(c, W_R) : 0.f, fx, fy, fz);
+        cell_force_flat(nb, se, N, dx, live, live ? me.x : 0.f, live ? me.y : 0.f, live ? me.z : 0.f,
+                        live ? me.w : 0.f, fx, fy, fz);
     } else if (VARIANT == 3) {
-        if (live) cell_force_grouped(nb, se, N, dx, cf(c, W_PX), cf(c, W_PY), cf(c, W_PZ), cf(c, W_R), fx, fy, fz);
+        if (live) cell_force_grouped(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     } else if (live) {
-        cell_force<VARIANT == 1>(nb, se, N, dx, cf(c, W_PX), cf(c, W_PY), cf(c, W_PZ), cf(c, W_R), fx, fy, fz);
+        cell_force<VARIANT == 1>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     }
     if (!in_range) return;
+    CellRegs c = load_cell(sorted, i);
     if (live) move_cell(c, fx, fy, fz, N, dx, dt);
     store_cell(cells, i, c);
 }

@@ -20,7 +20,8 @@
 //                  wave access; the whole line stays in registers.
 //   lod_x_flat<N>  the x sweep for N = 100 or 200 (not multiples of the
 //                  32-voxel chunk): 64 lines staged whole through LDS.
-//   lod_xy_plane<N> x and y sweeps of one z plane in one workgroup (N = 100).
+//   lod_xy_plane<N> x and y sweeps of one z plane in one workgroup (N = 100);
+//   lod_xy_plane_part<N, P> the same with partitioned line solves.
 //   lod_sweep_any  any other N: one lane per line, Y kept in place in the
 //                  field.
 // Bytes per voxel-update (algorithmic): x 12 (p, p_rate in; p out), y 8, z 8.
@@ -368,7 +369,8 @@ __device__ __forceinline__ void part_local(float (&v)[m], const float* __restric
     for (int i = m - 2; i >= 0; --i) v[i] = (v[i] - side * v[i + 1]) * iFl[i];
 }
 
-template <int N, int P>
+// bt, bb: P x LS floats of LDS (LS lines per block of the launch).
+template <int N, int P, int LS = 64>
 __device__ __forceinline__ void part_solve(float (&v)[N / P], const float* __restrict__ tab, int k, float side,
                                            float bnd, float* __restrict__ bt, float* __restrict__ bb, int lane) {
     constexpr int m = N / P;
@@ -377,24 +379,24 @@ __device__ __forceinline__ void part_solve(float (&v)[N / P], const float* __res
     const float* Rinv = tab + 4 * N;
     part_local<m>(v, tab + k * m, tab + N + k * m, side);
     // exchange block boundary values through LDS
-    bt[k * 64 + lane] = v[0];
-    bb[k * 64 + lane] = v[m - 1];
+    bt[k * LS + lane] = v[0];
+    bb[k * LS + lane] = v[m - 1];
     __syncthreads();
     float um1 = 0.f, tp1 = 0.f;
     if (k > 0) {
         const float* row = Rinv + (2 * (k - 1) + 1) * (2 * P);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            um1 += row[2 * q] * bt[q * 64 + lane];
-            um1 += row[2 * q + 1] * bb[q * 64 + lane];
+            um1 += row[2 * q] * bt[q * LS + lane];
+            um1 += row[2 * q + 1] * bb[q * LS + lane];
         }
     }
     if (k < P - 1) {
         const float* row = Rinv + (2 * (k + 1)) * (2 * P);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            tp1 += row[2 * q] * bt[q * 64 + lane];
-            tp1 += row[2 * q + 1] * bb[q * 64 + lane];
+            tp1 += row[2 * q] * bt[q * LS + lane];
+            tp1 += row[2 * q + 1] * bb[q * LS + lane];
         }
     }
 #pragma unroll
@@ -429,6 +431,98 @@ __global__ __launch_bounds__(64 * P) void lod_yz_part(float* __restrict__ p, con
 #pragma unroll
     for (int i = 0; i < m; ++i)
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[i]), rs, voff, so0 + i * stride_b, 0);
+}
+
+// x and y sweeps of one z plane with partitioned line solves (P blocks per
+// line of m = N / P voxels), for N whose plane fits in LDS and is not a
+// multiple of the 32-voxel chunk (N = 100). The plane is read once as in
+// lod_xy_plane; thread (k, j) solves block k of x line j out of the odd-pitch
+// tile and the blocks are joined through the reduced system (part_solve),
+// then the same for the y lines (thread (k, i): block k of y line i), and the
+// plane is written once. Rounding differs from the serial sweep at the 1e-7
+// relative level, as for lod_x_part / lod_yz_part; each chain is N / P long
+// instead of N, and P times as many lanes work on a plane.
+template <int N, int P>
+__global__ __launch_bounds__(128 * P) void lod_xy_plane_part(float* __restrict__ p, const float* __restrict__ rate,
+                                                             const float* __restrict__ tab, float side, float bnd,
+                                                             float dt) {
+    static_assert(N % 4 == 0 && N <= 128 && N % P == 0, "plane must fit in LDS, lines split evenly");
+    constexpr int m = N / P;
+    constexpr int Q = N / 4;
+    constexpr int NQ = N * Q;
+    constexpr int T = 128 * P;
+    constexpr int PER = (NQ + T - 1) / T;
+    constexpr int kB = 10;
+    constexpr int LD = N | 1;
+    __shared__ float tile[N * LD];
+    __shared__ float bt[P * 128], bb[P * 128];
+    const int tid = threadIdx.x;
+    const int k = __builtin_amdgcn_readfirstlane(tid >> 7);
+    const int j = tid & 127;
+    const int kp = blockIdx.x;
+    float* __restrict__ plane = p + (int64_t)kp * N * N;
+    const float4* __restrict__ pb = reinterpret_cast<const float4*>(plane);
+    const float4* __restrict__ rb = reinterpret_cast<const float4*>(rate + (int64_t)kp * N * N);
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += kB) {
+        float4 a[kB], r[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * T + tid;
+            if (b0 + u < PER && q < NQ) {
+                a[u] = pb[q];
+                r[u] = rb[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int q = (b0 + u) * T + tid;
+            if (b0 + u < PER && q < NQ) {
+                const int jj = q / Q, i0 = 4 * (q - jj * Q);
+                const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+                const float rv[4] = {r[u].x, r[u].y, r[u].z, r[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float x = av[e];
+                    if (on_boundary(i0 + e, jj, kp, N)) x = bnd;
+                    tile[jj * LD + i0 + e] = x / (1.f - rv[e] * dt);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const bool act = j < N;
+    float v[m];
+#pragma unroll
+    for (int i = 0; i < m; ++i) v[i] = act ? tile[j * LD + k * m + i] : 0.f;  // x line j, block k
+    if (k == 0) v[0] = bnd;
+    if (k == P - 1) v[m - 1] = bnd;
+    part_solve<N, P, 128>(v, tab, k, side, bnd, bt, bb, j);
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < m; ++i) tile[j * LD + k * m + i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < m; ++t) v[t] = act ? tile[(k * m + t) * LD + j] : 0.f;  // y line j, block k
+    if (k == 0) v[0] = bnd;
+    if (k == P - 1) v[m - 1] = bnd;
+    part_solve<N, P, 128>(v, tab, k, side, bnd, bt, bb, j);
+    if (act) {
+#pragma unroll
+        for (int t = 0; t < m; ++t) tile[(k * m + t) * LD + j] = v[t];
+    }
+    __syncthreads();
+    float4* __restrict__ po = reinterpret_cast<float4*>(plane);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int q = u * T + tid;
+        if (q < NQ) {
+            const int jj = q / Q, i0 = 4 * (q - jj * Q);
+            const float* src = &tile[jj * LD + i0];
+            po[q] = make_float4(src[0], src[1], src[2], src[3]);
+        }
+    }
 }
 
 // x sweep with fused boundary + uptake for the 64 x lines L0.. (P waves per
@@ -755,6 +849,30 @@ void launch_yz_part(float* p, const float* tab, float side, float bnd, hipStream
     hipLaunchKernelGGL((lod_yz_part<N, DIR, P>), dim3(blocks), dim3(64 * P), 0, st, p, tab, side, bnd);
 }
 
+// N = 100: the x and y sweeps run per z plane (xp = 2 or 4: partitioned
+// blocks; 1: serial lines, lod_xy_plane), the z sweep partitioned (yzp = 2 or
+// 4, lod_yz_part) or serial (lod_yz_reg).
+template <int N>
+hipError_t launch_plane_part(float* p, const float* rate, const float* E, const float* F, float* const* tabs,
+                             float side, float bnd, float dt, int xp, int yzp, int spill, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
+    if (xp >= 4)
+        hipLaunchKernelGGL((lod_xy_plane_part<N, 4>), dim3(N), dim3(512), 0, st, p, rate, tabs[1], side, bnd, dt);
+    else if (xp >= 2)
+        hipLaunchKernelGGL((lod_xy_plane_part<N, 2>), dim3(N), dim3(256), 0, st, p, rate, tabs[0], side, bnd, dt);
+    else
+        hipLaunchKernelGGL((lod_xy_plane<N>), dim3(N), dim3(128), 0, st, p, rate, E, F, side, bnd, dt);
+    if (yzp >= 4)
+        launch_yz_part<N, 2, 4>(p, tabs[1], side, bnd, st);
+    else if (yzp >= 2)
+        launch_yz_part<N, 2, 2>(p, tabs[0], side, bnd, st);
+    else if (spill > 0)
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    else
+        hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
+    return hipGetLastError();
+}
+
 // One substep with per-sweep choices: xp / yzp = 1 (serial line per lane),
 // 2, 4, 8 or 16 (partitioned; a choice the grid does not support -- N / P
 // not a whole number of 32-voxel chunks -- falls back to the next smaller
@@ -875,7 +993,10 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
         case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, spill, st);
         case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, spill, st);
         case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, spill, st);
-        case 100: return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, ctx->tune[GELL_KNOB_LOD_FUSED], st);
+        case 100:
+            if (xp > 1 || yzp > 1)
+                return launch_plane_part<100>(p, rate, E, F, ctx->ptab[s], side, bnd, dt, xp, yzp, spill, st);
+            return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, ctx->tune[GELL_KNOB_LOD_FUSED], st);
         case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, false, st);
         default: break;
     }
@@ -1013,8 +1134,13 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
     if (part_ok && ctx->coup[0])
         for (int yi = 1; yi < 5; ++yi)
             if (N % (cand[yi] * 32) == 0) cands.push_back({N / 64, cand[yi], 0, 1});
-    if (N == 100)  // x + y per z plane (serial lines, so also bit-exact)
+    if (N == 100) {  // x + y per z plane: serial lines (bit-exact), then partitioned blocks
         for (int sp = 0; sp < 2; ++sp) cands.push_back({1, 1, sp, 1});
+        if (!bit_exact)
+            for (int xp : {1, 2, 4})
+                for (int yzp : {1, 2, 4})
+                    if (xp > 1 || yzp > 1) cands.push_back({xp, yzp, 0, 1});
+    }
     const int knobs[4] = {GELL_KNOB_LOD_X_PARTS, GELL_KNOB_LOD_PARTS, GELL_KNOB_LOD_YZ_LDS, GELL_KNOB_LOD_FUSED};
     int saved[4], best[4];
     for (int q = 0; q < 4; ++q) saved[q] = best[q] = ctx->tune[knobs[q]];
@@ -1079,7 +1205,7 @@ hipError_t lod_init_coeffs(gell_ctx* ctx, int s) {
     const int parts[4] = {2, 4, 8, 16};
     std::vector<float> tabs[4];
     for (int q = 0; q < 4; ++q) {
-        if (N % (parts[q] * kChunk) != 0) continue;
+        if (N % parts[q] != 0 || N / parts[q] < 2) continue;
         tabs[q] = part_table(N, parts[q], side64, corner64, diag64);
         hipError_t e = hipMemcpyAsync(ctx->ptab[s][q], tabs[q].data(), tabs[q].size() * sizeof(float),
                                       hipMemcpyHostToDevice, ctx->stream);

@@ -191,9 +191,14 @@ int gell_sync(gell_ctx* ctx);
  *                              solve, rounding differs at the 1e-7 relative
  *                              level); applies for N in {128, 192, 256, 512},
  *                              a block count the grid cannot split into
- *                              32-voxel chunks falls back to a smaller one.
+ *                              32-voxel chunks falls back to a smaller one;
+ *                              at N = 100 it is the z sweep's block count
+ *                              (2 or 4).
  *                              Default 4.
- *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep (at most 8).
+ *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep (at most 8); at
+ *                              N = 100 the block count (1, 2 or 4) of the x
+ *                              and y sweeps, which run per z plane in one
+ *                              workgroup.
  *                              Default 4.
  *   GELL_KNOB_LOD_FUSED    1 = for N in {128, 256, 512} with LOD_PARTS > 1,
  *                              the x sweep and the y blocks run in one pass
