@@ -20,8 +20,9 @@
 //                  wave access; the whole line stays in registers.
 //   lod_x_flat<N>  the x sweep for N = 100 or 200 (not multiples of the
 //                  32-voxel chunk): 64 lines staged whole through LDS.
-//   lod_xy_plane<N> x and y sweeps of one z plane in one workgroup (N = 100);
-//   lod_xy_plane_part<N, P> the same with partitioned line solves.
+//   lod_xy_plane<N> x and y sweeps of one z plane in one workgroup (N = 64,
+//                  96, 100); lod_xy_plane_part<N, P> the same with
+//                  partitioned line solves.
 //   lod_sweep_any  any other N: one lane per line, Y kept in place in the
 //                  field.
 // Bytes per voxel-update (algorithmic): x 12 (p, p_rate in; p out), y 8, z 8.
@@ -849,9 +850,9 @@ void launch_yz_part(float* p, const float* tab, float side, float bnd, hipStream
     hipLaunchKernelGGL((lod_yz_part<N, DIR, P>), dim3(blocks), dim3(64 * P), 0, st, p, tab, side, bnd);
 }
 
-// N = 100: the x and y sweeps run per z plane (xp = 2 or 4: partitioned
-// blocks; 1: serial lines, lod_xy_plane), the z sweep partitioned (yzp = 2 or
-// 4, lod_yz_part) or serial (lod_yz_reg).
+// N in {64, 96, 100}: the x and y sweeps run per z plane (xp = 2 or 4:
+// partitioned blocks; 1: serial lines, lod_xy_plane), the z sweep
+// partitioned (yzp = 2 or 4, lod_yz_part) or serial (lod_yz_reg).
 template <int N>
 hipError_t launch_plane_part(float* p, const float* rate, const float* E, const float* F, float* const* tabs,
                              float side, float bnd, float dt, int xp, int yzp, int spill, hipStream_t st) {
@@ -973,6 +974,7 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     const int spill = ctx->tune[GELL_KNOB_LOD_YZ_LDS];
     const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
+    const bool fused = ctx->tune[GELL_KNOB_LOD_FUSED] != 0;
     if (lod_fused_active(ctx)) {
         float* const* tabs = ctx->ptab[s];
         if (N == 128) return launch_fused<128>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
@@ -988,15 +990,21 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     }
     switch (N) {
         case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, spill, st);
-        case 64: return launch_reg<64>(p, rate, E, F, side, bnd, dt, spill, st);
-        case 96: return launch_reg<96>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 64:
+            if (xp > 1 || yzp > 1 || fused)
+                return launch_plane_part<64>(p, rate, E, F, ctx->ptab[s], side, bnd, dt, xp, yzp, spill, st);
+            return launch_reg<64>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 96:
+            if (xp > 1 || yzp > 1 || fused)
+                return launch_plane_part<96>(p, rate, E, F, ctx->ptab[s], side, bnd, dt, xp, yzp, spill, st);
+            return launch_reg<96>(p, rate, E, F, side, bnd, dt, spill, st);
         case 128: return launch_reg<128>(p, rate, E, F, side, bnd, dt, spill, st);
         case 192: return launch_reg<192>(p, rate, E, F, side, bnd, dt, spill, st);
         case 256: return launch_reg<256>(p, rate, E, F, side, bnd, dt, spill, st);
         case 100:
-            if (xp > 1 || yzp > 1)
+            if (xp > 1 || yzp > 1 || fused)
                 return launch_plane_part<100>(p, rate, E, F, ctx->ptab[s], side, bnd, dt, xp, yzp, spill, st);
-            return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, ctx->tune[GELL_KNOB_LOD_FUSED], st);
+            return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, false, st);
         case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, false, st);
         default: break;
     }
@@ -1134,7 +1142,7 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us) {
     if (part_ok && ctx->coup[0])
         for (int yi = 1; yi < 5; ++yi)
             if (N % (cand[yi] * 32) == 0) cands.push_back({N / 64, cand[yi], 0, 1});
-    if (N == 100) {  // x + y per z plane: serial lines (bit-exact), then partitioned blocks
+    if (N == 64 || N == 96 || N == 100) {  // x + y per z plane: serial lines (bit-exact), then partitioned blocks
         for (int sp = 0; sp < 2; ++sp) cands.push_back({1, 1, sp, 1});
         if (!bit_exact)
             for (int xp : {1, 2, 4})

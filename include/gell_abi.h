@@ -192,13 +192,13 @@ int gell_sync(gell_ctx* ctx);
  *                              level); applies for N in {128, 192, 256, 512},
  *                              a block count the grid cannot split into
  *                              32-voxel chunks falls back to a smaller one;
- *                              at N = 100 it is the z sweep's block count
- *                              (2 or 4).
+ *                              at N = 64, 96 and 100 it is the z sweep's
+ *                              block count (2 or 4).
  *                              Default 4.
  *   GELL_KNOB_LOD_X_PARTS  the same choice for the x sweep (at most 8); at
- *                              N = 100 the block count (1, 2 or 4) of the x
- *                              and y sweeps, which run per z plane in one
- *                              workgroup.
+ *                              N = 64, 96 and 100 the block count (1, 2 or 4)
+ *                              of the x and y sweeps, which run per z plane
+ *                              in one workgroup.
  *                              Default 4.
  *   GELL_KNOB_LOD_FUSED    1 = for N in {128, 256, 512} with LOD_PARTS > 1,
  *                              the x sweep and the y blocks run in one pass
@@ -207,10 +207,10 @@ int gell_sync(gell_ctx* ctx);
  *                              traffic per voxel instead of 28, x and y in
  *                              N / 64 blocks, z in LOD_PARTS blocks; results
  *                              equal the unfused kernels with those block
- *                              counts bit for bit. At N = 100 (serial
- *                              lines) 1 = the x and y sweeps of each z plane
- *                              in one workgroup, bit-identical to the
- *                              separate sweeps. Default 1.
+ *                              counts bit for bit. At N = 64, 96 and 100
+ *                              with serial lines, 1 = the x and y sweeps of
+ *                              each z plane in one workgroup, bit-identical
+ *                              to the separate sweeps. Default 1.
  *   GELL_KNOB_SUBSTRATE_STREAMS 1 = each substrate's LOD substeps run on a
  *                              stream of its own, overlapping the other
  *                              substrates' substeps and the next mechanics

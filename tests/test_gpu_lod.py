@@ -47,15 +47,15 @@ def test_lod_substeps_match_oracle(N, oracle):
     assert u == 0
 
 
+@pytest.mark.parametrize("N", [64, 96, 100])
 @pytest.mark.parametrize("plane", [0, 1])
 @pytest.mark.parametrize("spill", [0, 1])
-def test_lod_100_register_and_lds_variants(spill, plane, oracle):
-    """N = 100 (the reference's Voxel_num): the LDS-staged x sweep and the
-    register y/z sweeps, or the x + y sweeps per z plane in one workgroup,
-    with or without the LDS spill of the z sweep: all bit-exact."""
-    N = 100
+def test_lod_serial_plane_and_line_variants(N, spill, plane, oracle):
+    """Serial lines at N = 64, 96 and 100 (the reference's Voxel_num): the
+    separate x, y, z kernels, or the x + y sweeps per z plane in one
+    workgroup, with or without the LDS spill of the z sweep: all bit-exact."""
     P = params(N, dx=25.0)
-    p, rate = _field_and_rate(N, 7 + spill + 2 * plane)
+    p, rate = _field_and_rate(N, N + spill + 2 * plane)
     st = GpuState(P, np.zeros(0, rt.CELL_DTYPE), exact=True)
     st.ctx.tune("lod_yz_lds", spill)
     st.ctx.tune("lod_fused", plane)

@@ -80,7 +80,8 @@ def test_bad_knob_is_an_error():
     st.close()
 
 
-@pytest.mark.parametrize("N,xp,yzp", [(100, 2, 2), (100, 4, 4), (100, 4, 1), (100, 1, 4), (100, 2, 4),
+@pytest.mark.parametrize("N,xp,yzp", [(64, 2, 2), (64, 4, 4), (96, 4, 2), (100, 2, 2), (100, 4, 4), (100, 4, 1),
+                                      (100, 1, 4), (100, 2, 4),
                                       (128, 2, 2), (128, 4, 4), (192, 2, 2), (256, 2, 1), (256, 1, 2),
                                       (256, 4, 4), (256, 2, 4), (256, 8, 16), (512, 4, 4), (512, 8, 8),
                                       (512, 8, 16), (512, 1, 1)])
