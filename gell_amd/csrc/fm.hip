@@ -260,6 +260,103 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint32_t* __
     }
 }
 
+// Small counts: the whole sort in one workgroup and one launch. The tiled
+// passes above cost three launches each, which at a few thousand keys is
+// most of the sort's time. Same stable LSD order: within a wave, ranks come
+// from the digit ballots in index order; wave w holds keys [w kSmallPer,
+// (w + 1) kSmallPer); per digit the waves are offset in order. Between passes
+// the keys and values go through the two global ping-pong buffers (the
+// workgroup barrier makes one pass's writes visible to the next).
+constexpr int kSmallThreads = 1024, kSmallItems = 16, kSmallWaves = kSmallThreads / 64;
+constexpr int kSmallPer = 64 * kSmallItems;
+constexpr int kSmallMax = kSmallThreads * kSmallItems;
+
+__device__ __forceinline__ uint64_t match_digit_rt(uint32_t d, uint64_t valid, int bits) {
+    uint64_t peers = valid;
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __restrict__ k0, int32_t* __restrict__ v0,
+                                                                  uint32_t* __restrict__ k1, int32_t* __restrict__ v1,
+                                                                  int32_t* __restrict__ order_out,
+                                                                  int32_t* __restrict__ signed_out, int32_t n,
+                                                                  SortPlan plan) {
+    __shared__ int cnt[kSmallWaves][512];
+    __shared__ int dbase[512];
+    __shared__ int lds16[kSmallWaves];
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / 64;
+    uint32_t* kin = k0;
+    uint32_t* kout = k1;
+    int32_t* vin = nullptr;
+    int32_t* vbuf[2] = {v0, v1};
+    int cur = 0;
+    for (int q = 0; q < plan.passes; ++q) {
+        const int bits = plan.bits[q], shift = plan.shift[q], bins = 1 << bits;
+        const bool last = q == plan.passes - 1;
+        int32_t* vout = last ? order_out : vbuf[1 - cur];
+        for (int b = tid; b < kSmallWaves * 512; b += kSmallThreads) (&cnt[0][0])[b] = 0;
+        __syncthreads();
+        uint32_t k[kSmallItems];
+        int32_t val[kSmallItems];
+        int rk[kSmallItems];
+#pragma unroll
+        for (int it = 0; it < kSmallItems; ++it) {
+            const int idx = w * kSmallPer + it * 64 + lane;
+            const bool valid = idx < n;
+            // agent-scope loads read L2, where the previous pass's stores went
+            // (the vector L1 may still hold lines of this buffer from two passes ago)
+            k[it] = valid ? __hip_atomic_load(&kin[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+            val[it] = valid ? (vin ? __hip_atomic_load(&vin[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : idx)
+                            : 0;
+            const uint32_t d = (k[it] >> shift) & (uint32_t)(bins - 1);
+            const uint64_t peers = match_digit_rt(d, __ballot(valid), bits);
+            const int before = valid ? cnt[w][d] : 0;
+            rk[it] = before + __popcll(peers & lanemask_lt());
+            if (valid && lane == 63 - __clzll(peers)) cnt[w][d] = before + __popcll(peers);
+        }
+        __syncthreads();
+        // per digit: offsets of the waves (in place) and the digit's total
+        int acc = 0;
+        if (tid < bins) {
+            for (int ww = 0; ww < kSmallWaves; ++ww) {
+                const int c = cnt[ww][tid];
+                cnt[ww][tid] = acc;
+                acc += c;
+            }
+        }
+        // exclusive scan of the totals over digits (thread = digit)
+        const int inc = wave_inclusive_scan(acc);
+        if (lane == 63) lds16[w] = inc;
+        __syncthreads();
+        int off = 0;
+        for (int ww = 0; ww < w; ++ww) off += lds16[ww];
+        if (tid < bins) dbase[tid] = off + inc - acc;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < kSmallItems; ++it) {
+            const int idx = w * kSmallPer + it * 64 + lane;
+            if (idx < n) {
+                const uint32_t d = (k[it] >> shift) & (uint32_t)(bins - 1);
+                const int pos = dbase[d] + cnt[w][d] + rk[it];
+                kout[pos] = k[it];
+                vout[pos] = val[it];
+                if (last) signed_out[pos] = (int32_t)k[it] - 1;
+            }
+        }
+        __syncthreads();
+        uint32_t* t = kin;
+        kin = kout;
+        kout = t;
+        vin = vout;
+        cur = 1 - cur;
+    }
+}
+
 template <int BITS>
 hipError_t sort_pass(gell_ctx* ctx, const uint32_t* kin, const int32_t* vin, uint32_t* kout, int32_t* vout,
                      int32_t* signed_out, int32_t n, int shift) {
@@ -784,6 +881,11 @@ hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
     if (n == 0) return hipSuccess;
     const SortPlan plan = plan_sort(3 * ceil_log2(ctx->N) + 1);
+    if (n <= kSmallMax) {
+        hipLaunchKernelGGL(radix_sort_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ctx->keys[0], ctx->vals[0],
+                           ctx->keys[1], ctx->vals[1], ctx->out_order, ctx->out_keys, n, plan);
+        return hipGetLastError();
+    }
     int cur = 0;
     for (int q = 0; q < plan.passes; ++q) {
         const bool last = q == plan.passes - 1;

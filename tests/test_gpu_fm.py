@@ -31,7 +31,8 @@ def _check_step(st: GpuState, cells: np.ndarray, n: int, oracle, label: str):
     return want
 
 
-@pytest.mark.parametrize("N,n,dense", [(32, 2000, True), (64, 10000, False), (64, 6000, True), (96, 20000, True)])
+@pytest.mark.parametrize("N,n,dense", [(32, 2000, True), (64, 10000, False), (64, 6000, True), (64, 16384, False),
+                                       (64, 16385, False), (96, 20000, True)])
 def test_fm_steps_match_oracle(N, n, dense, oracle):
     log(f"[fm] N={N} n={n} dense={dense}")
     P = params(N, capacity=n + 64)
