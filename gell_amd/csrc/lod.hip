@@ -546,6 +546,22 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
     const int lq = lane / Q, part = lane % Q;
     const int voff = (lq * N + part * 4) * 4;
     const int sbase = (int)(L0 * N * 4) + k * m * 4;
+    // With at most two chunks every load of the segment is issued before the
+    // first transpose, so the second chunk does not wait for a round trip of
+    // its own behind the first chunk's barriers (the registers are free: LDS
+    // already limits these kernels to two waves per SIMD).
+    constexpr bool kPre = NC <= 2;
+    float4 pa[kPre ? NC : 1][Q], pr[kPre ? NC : 1][Q];
+    if constexpr (kPre) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int it = 0; it < Q; ++it) {
+                const int so = sbase + (it * LPI * N + c * kChunk) * 4;
+                pa[c][it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
+                pr[c][it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+            }
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -555,8 +571,14 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
             const int j = (int)(L % N), kk = (int)(L / N);
             const int i0 = k * m + c * kChunk + part * 4;
             const int so = sbase + (it * LPI * N + c * kChunk) * 4;
-            const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
-            const float4 r = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+            float4 a, r;
+            if constexpr (kPre) {
+                a = pa[c][it];
+                r = pr[c][it];
+            } else {
+                a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
+                r = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+            }
             const float av[4] = {a.x, a.y, a.z, a.w};
             const float rv[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
