@@ -122,6 +122,7 @@ void release(gell_ctx* c) {
         dfree(c->ptab[s][3]);
         dfree(c->acc[s]);
         dfree(c->rate_list[s]);
+        dfree(c->xspan[s]);
     }
     for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) dfree(c->coup[s]);
     dfree(c->flags);
@@ -231,6 +232,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         A(dalloc(&c->ptab[s][3], (size_t)(4 * P.N + 1024)));
         A(dalloc(&c->acc[s], (size_t)c->V));
         A(dalloc(&c->rate_list[s], cap));
+        A(dalloc(&c->xspan[s], (size_t)P.N * P.N));
     }
     A(dalloc(&c->flags, cap));
     A(dalloc(&c->blk, (size_t)c->blk_cap));
@@ -256,6 +258,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         hipLaunchKernelGGL(fill_f32, dim3(vb), dim3(256), 0, c->stream, c->field[s], c->V, P.sub[s].initial);
         A(hipMemsetAsync(c->rate[s], 0, sizeof(float) * (size_t)c->V, c->stream));
         A(hipMemsetAsync(c->acc[s], 0, sizeof(long long) * (size_t)c->V, c->stream));
+        A(gell::xspan_rebuild(c, s, c->stream));  // all empty: the map is zero
         A(gell::lod_init_coeffs(c, s));
     }
     A(gell::fm_init_ranges(c));
@@ -380,7 +383,9 @@ float* gell_field(gell_ctx* ctx, int32_t s) {
     return (ctx && s >= 0 && s < ctx->P.n_substrates) ? ctx->field[s] : nullptr;
 }
 float* gell_uptake_field(gell_ctx* ctx, int32_t s) {
-    return (ctx && s >= 0 && s < ctx->P.n_substrates) ? ctx->rate[s] : nullptr;
+    if (!ctx || s < 0 || s >= ctx->P.n_substrates) return nullptr;
+    ctx->xspan_valid[s] = false;  // the caller may write the map: rebuild the line spans before use
+    return ctx->rate[s];
 }
 const int32_t* gell_sorted_keys(gell_ctx* ctx) { return ctx ? ctx->out_keys : nullptr; }
 const int32_t* gell_sorted_order(gell_ctx* ctx) { return ctx ? ctx->out_order : nullptr; }

@@ -39,6 +39,12 @@ struct gell_ctx {
     // Diffusion, per substrate.
     float* field[GELL_MAX_SUBSTRATES] = {};
     float* rate[GELL_MAX_SUBSTRATES] = {};
+    // Per x line (j + N k): first / last voxel index that may hold a nonzero
+    // uptake rate ((N, -1): none). Kept by PreLOD; the partitioned x sweep
+    // reads no rate outside it (a zero rate divides by exactly 1). Valid while
+    // no caller had the uptake-map pointer since it was last rebuilt.
+    int2* xspan[GELL_MAX_SUBSTRATES] = {};
+    bool xspan_valid[GELL_MAX_SUBSTRATES] = {};
     float* E[GELL_MAX_SUBSTRATES] = {};
     float* F[GELL_MAX_SUBSTRATES] = {};
     float side[GELL_MAX_SUBSTRATES] = {};
@@ -91,6 +97,7 @@ enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, P
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);
+hipError_t xspan_rebuild(gell_ctx* ctx, int s, hipStream_t st);
 // Make the context stream wait for substrate s's own stream (no-op when
 // nothing is pending there); join_substrates does it for every substrate.
 hipError_t join_substrate(gell_ctx* ctx, int s);

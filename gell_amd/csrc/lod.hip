@@ -529,12 +529,14 @@ __global__ __launch_bounds__(128 * P) void lod_xy_plane_part(float* __restrict__
 // x sweep with fused boundary + uptake for the 64 x lines L0.. (P waves per
 // line, wave k owning columns [k m, (k + 1) m)); each wave loads its segment
 // through its LDS transpose tile and leaves its solved segment in v.
+// xspan (per x line, see gell_ctx) limits the rate loads to the voxels that
+// may hold a nonzero rate; the others use 0, which is what they hold.
 template <int N, int P>
 __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __restrict__ p,
-                                              const float* __restrict__ rate, const float* __restrict__ tab,
-                                              float side, float bnd, float dt, int64_t L0, int k, int lane,
-                                              float* __restrict__ tile, float* __restrict__ bt,
-                                              float* __restrict__ bb) {
+                                              const float* __restrict__ rate, const int2* __restrict__ xspan,
+                                              const float* __restrict__ tab, float side, float bnd, float dt,
+                                              int64_t L0, int k, int lane, float* __restrict__ tile,
+                                              float* __restrict__ bt, float* __restrict__ bb) {
     constexpr int m = N / P;
     static_assert(m % kChunk == 0, "segment must be a whole number of chunks");
     constexpr int NC = m / kChunk;
@@ -551,6 +553,12 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
     // its own behind the first chunk's barriers (the registers are free: LDS
     // already limits these kernels to two waves per SIMD).
     constexpr bool kPre = NC <= 2;
+    int2 sp[Q];
+#pragma unroll
+    for (int it = 0; it < Q; ++it) sp[it] = xspan[L0 + it * LPI + lq];
+    // the lane's four voxels [i0, i0 + 3] of line `it` may hold a nonzero rate
+    auto rate_needed = [&](int it, int i0) { return i0 <= sp[it].y && i0 + 3 >= sp[it].x; };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 pa[kPre ? NC : 1][Q], pr[kPre ? NC : 1][Q];
     if constexpr (kPre) {
 #pragma unroll
@@ -559,7 +567,9 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
             for (int it = 0; it < Q; ++it) {
                 const int so = sbase + (it * LPI * N + c * kChunk) * 4;
                 pa[c][it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
-                pr[c][it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+                pr[c][it] = rate_needed(it, k * m + c * kChunk + part * 4)
+                                ? __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0))
+                                : zero4;
             }
     }
 #pragma unroll
@@ -577,7 +587,8 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
                 r = pr[c][it];
             } else {
                 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rp, voff, so, 0));
-                r = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0));
+                r = rate_needed(it, i0) ? __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rr, voff, so, 0))
+                                        : zero4;
             }
             const float av[4] = {a.x, a.y, a.z, a.w};
             const float rv[4] = {r.x, r.y, r.z, r.w};
@@ -601,7 +612,8 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
 
 template <int N, int P>
 __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, const float* __restrict__ rate,
-                                                     const float* __restrict__ tab, float side, float bnd, float dt) {
+                                                     const int2* __restrict__ xspan, const float* __restrict__ tab,
+                                                     float side, float bnd, float dt) {
     constexpr int m = N / P;
     constexpr int NC = m / kChunk;
     constexpr int Q = kChunk / 4;
@@ -614,7 +626,7 @@ __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, cons
     const int64_t L0 = (int64_t)blockIdx.x * 64;
     const int lq = lane / Q, part = lane % Q;
     float v[m];
-    x_lines_solve<N, P>(v, p, rate, tab, side, bnd, dt, L0, k, lane, tile, bt, bb);
+    x_lines_solve<N, P>(v, p, rate, xspan, tab, side, bnd, dt, L0, k, lane, tile, bt, bb);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -653,8 +665,8 @@ __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, cons
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const float* __restrict__ rate,
-                                                 const float* __restrict__ tab, float side, float bnd, float dt,
-                                                 float* __restrict__ coup) {
+                                                 const int2* __restrict__ xspan, const float* __restrict__ tab,
+                                                 float side, float bnd, float dt, float* __restrict__ coup) {
     constexpr int P = N / 64;
     constexpr int m = 64;  // x segment per wave = y rows per block
     constexpr int64_t NN = (int64_t)N * N;
@@ -668,7 +680,7 @@ __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const fl
     const int64_t L0 = (int64_t)blockIdx.x * 64;  // x lines (j, k), j = L % N
     const int kz = (int)(L0 / N), yb = (int)((L0 % N) / 64);
     float v[m];
-    x_lines_solve<N, P>(v, p, rate, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
+    x_lines_solve<N, P>(v, p, rate, xspan, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
     // every wave is past its transpose-tile reads (part_solve synchronises)
 #pragma unroll
     for (int t = 0; t < m; ++t) smem[lane * RowLd + kx * m + t] = v[t];
@@ -861,9 +873,10 @@ hipError_t launch_reg_flat(float* p, const float* rate, const float* E, const fl
 }
 
 template <int N, int P>
-void launch_x_part(float* p, const float* rate, const float* tab, float side, float bnd, float dt, hipStream_t st) {
+void launch_x_part(float* p, const float* rate, const int2* xspan, const float* tab, float side, float bnd, float dt,
+                   hipStream_t st) {
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
-    hipLaunchKernelGGL((lod_x_part<N, P>), dim3(blocks), dim3(64 * P), 0, st, p, rate, tab, side, bnd, dt);
+    hipLaunchKernelGGL((lod_x_part<N, P>), dim3(blocks), dim3(64 * P), 0, st, p, rate, xspan, tab, side, bnd, dt);
 }
 
 template <int N, int DIR, int P>
@@ -904,18 +917,18 @@ hipError_t launch_plane_part(float* p, const float* rate, const float* E, const 
 // N = 256 a whole line does not fit in registers and the serial choice runs
 // the generic kernel.
 template <int N>
-hipError_t launch_mixed(float* p, const float* rate, const float* E, const float* F, float* const* tabs,
+hipError_t launch_mixed(float* p, const float* rate, const int2* xspan, const float* E, const float* F, float* const* tabs,
                         float side, float bnd, float dt, int xp, int yzp, int spill, hipStream_t st) {
     constexpr bool ok2 = N % (2 * kChunk) == 0, ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0,
                    ok16 = N % (16 * kChunk) == 0;
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
     const unsigned gblocks = (unsigned)(((int64_t)N * N + 255) / 256);
     if (xp >= 8 && ok8)
-        launch_x_part<N, ok8 ? 8 : 2>(p, rate, tabs[2], side, bnd, dt, st);
+        launch_x_part<N, ok8 ? 8 : 2>(p, rate, xspan, tabs[2], side, bnd, dt, st);
     else if (xp >= 4 && ok4)
-        launch_x_part<N, ok4 ? 4 : 2>(p, rate, tabs[1], side, bnd, dt, st);
+        launch_x_part<N, ok4 ? 4 : 2>(p, rate, xspan, tabs[1], side, bnd, dt, st);
     else if (xp >= 2 && ok2)
-        launch_x_part<N, 2>(p, rate, tabs[0], side, bnd, dt, st);
+        launch_x_part<N, 2>(p, rate, xspan, tabs[0], side, bnd, dt, st);
     else if constexpr (N <= 256)
         hipLaunchKernelGGL((lod_x_reg<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
     else
@@ -957,13 +970,13 @@ void launch_z_ycorr(float* p, const float* tabz, const float* taby, const float*
 // Fused substep (see lod_xy_part); zp = z blocks, falling back to the next
 // smaller count the grid supports.
 template <int N>
-hipError_t launch_fused(float* p, const float* rate, float* const* tabs, float side, float bnd, float dt, int zp,
-                        float* coup, hipStream_t st) {
+hipError_t launch_fused(float* p, const float* rate, const int2* xspan, float* const* tabs, float side, float bnd,
+                        float dt, int zp, float* coup, hipStream_t st) {
     constexpr int P = N / 64;
     static_assert(P == 2 || P == 4 || P == 8, "fused substep needs N in {128, 256, 512}");
     const float* taby = tabs[P == 2 ? 0 : (P == 4 ? 1 : 2)];
-    hipLaunchKernelGGL((lod_xy_part<N>), dim3((unsigned)((int64_t)N * N / 64)), dim3(N), 0, st, p, rate, taby, side,
-                       bnd, dt, coup);
+    hipLaunchKernelGGL((lod_xy_part<N>), dim3((unsigned)((int64_t)N * N / 64)), dim3(N), 0, st, p, rate, xspan, taby,
+                       side, bnd, dt, coup);
     hipLaunchKernelGGL((lod_y_reduce<N>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0, st, taby,
                        coup);
     constexpr bool ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0, ok16 = N % (16 * kChunk) == 0;
@@ -997,18 +1010,23 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     const int yzp = ctx->tune[GELL_KNOB_LOD_PARTS];
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
     const bool fused = ctx->tune[GELL_KNOB_LOD_FUSED] != 0;
+    const int2* xspan = ctx->xspan[s];
+    if (!ctx->xspan_valid[s]) {
+        const hipError_t e = xspan_rebuild(ctx, s, st);
+        if (e != hipSuccess) return e;
+    }
     if (lod_fused_active(ctx)) {
         float* const* tabs = ctx->ptab[s];
-        if (N == 128) return launch_fused<128>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        if (N == 256) return launch_fused<256>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        return launch_fused<512>(p, rate, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 128) return launch_fused<128>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 256) return launch_fused<256>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        return launch_fused<512>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
     }
     if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256 || N == 512)) {
         float* const* tabs = ctx->ptab[s];
-        if (N == 128) return launch_mixed<128>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
-        if (N == 192) return launch_mixed<192>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
-        if (N == 256) return launch_mixed<256>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
-        return launch_mixed<512>(p, rate, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        if (N == 128) return launch_mixed<128>(p, rate, xspan, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        if (N == 192) return launch_mixed<192>(p, rate, xspan, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        if (N == 256) return launch_mixed<256>(p, rate, xspan, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
+        return launch_mixed<512>(p, rate, xspan, E, F, tabs, side, bnd, dt, xp, yzp, spill, st);
     }
     switch (N) {
         case 32: return launch_reg<32>(p, rate, E, F, side, bnd, dt, spill, st);

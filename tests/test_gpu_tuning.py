@@ -199,3 +199,38 @@ def test_substrate_streams_give_identical_results():
     for a, b in zip(outs[0][:2], outs[1][:2]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert np.array_equal(outs[0][2].view(np.uint8), outs[1][2].view(np.uint8))
+
+
+def test_rate_spans_kept_by_prelod_equal_spans_rebuilt_from_the_map(oracle):
+    """The partitioned x sweep reads uptake rates only inside each x line's
+    span of voxels that may hold a nonzero rate. Spans kept by PreLOD over
+    several mechanics steps (cells move, the previous map is cleared) must
+    give the same bits as spans rebuilt from the map before every substep
+    (handing out the uptake-map pointer forces the rebuild), and the field
+    must stay within the 1e-5 relative-L2 bar of the oracle (the partitioned
+    solve's rounding, under 1e-6 per substep, accumulates over 66 substeps)."""
+    from gell_amd import spheroid
+    from helpers import rel_l2
+
+    N, n = 128, 40000
+    P = params(N, capacity=n)
+    cells = spheroid.make_cells(n, N, 10.0, 420.0, 11, capacity=n)
+    outs = []
+    for rebuild in (False, True):
+        st = GpuState(P, cells)
+        assert st.ctx.tune_get("lod_fused") == 1 and st.ctx.tune_get("lod_parts") > 1
+        for _ in range(6):
+            st.ctx.fm_update(st.buf, n)
+            st.ctx.prelod(0, st.buf, n)
+            for _ in range(11):
+                if rebuild:
+                    st.ctx.lib.gell_uptake_field(st.ctx.h, 0)
+                st.ctx.lod_substep(0)
+        outs.append((st.cells()[:n].copy(), st.ctx.field(0)))
+        st.close()
+    assert outs[0][0].tobytes() == outs[1][0].tobytes()
+    assert outs[0][1].tobytes() == outs[1][1].tobytes()
+    want = cells.copy()
+    f = np.full(N ** 3, 38.0, np.float32)
+    oracle.run_mech(P, want, n, f, 6)
+    assert rel_l2(outs[0][1], f) < 1e-5
