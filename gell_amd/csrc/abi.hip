@@ -258,7 +258,6 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         hipLaunchKernelGGL(fill_f32, dim3(vb), dim3(256), 0, c->stream, c->field[s], c->V, P.sub[s].initial);
         A(hipMemsetAsync(c->rate[s], 0, sizeof(float) * (size_t)c->V, c->stream));
         A(hipMemsetAsync(c->acc[s], 0, sizeof(long long) * (size_t)c->V, c->stream));
-        A(gell::xspan_rebuild(c, s, c->stream));  // all empty: the map is zero
         A(gell::lod_init_coeffs(c, s));
     }
     A(gell::fm_init_ranges(c));

@@ -40,9 +40,10 @@ struct gell_ctx {
     float* field[GELL_MAX_SUBSTRATES] = {};
     float* rate[GELL_MAX_SUBSTRATES] = {};
     // Per x line (j + N k): first / last voxel index that may hold a nonzero
-    // uptake rate ((N, -1): none). Kept by PreLOD; the partitioned x sweep
-    // reads no rate outside it (a zero rate divides by exactly 1). Valid while
-    // no caller had the uptake-map pointer since it was last rebuilt.
+    // uptake rate ((N, -1): none); the partitioned x sweep reads no rate
+    // outside it (a zero rate divides by exactly 1). Rebuilt from the map by
+    // the first such substep after PreLOD or after the uptake-map pointer was
+    // handed out (the caller may have written the map).
     int2* xspan[GELL_MAX_SUBSTRATES] = {};
     bool xspan_valid[GELL_MAX_SUBSTRATES] = {};
     float* E[GELL_MAX_SUBSTRATES] = {};

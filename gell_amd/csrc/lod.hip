@@ -1011,7 +1011,9 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     const int xp = ctx->tune[GELL_KNOB_LOD_X_PARTS];
     const bool fused = ctx->tune[GELL_KNOB_LOD_FUSED] != 0;
     const int2* xspan = ctx->xspan[s];
-    if (!ctx->xspan_valid[s]) {
+    // the partitioned x sweep (fused, or xp > 1) reads the line spans of the uptake map
+    const bool spans = lod_fused_active(ctx) || (xp > 1 && (N == 128 || N == 192 || N == 256 || N == 512));
+    if (spans && !ctx->xspan_valid[s]) {
         const hipError_t e = xspan_rebuild(ctx, s, st);
         if (e != hipSuccess) return e;
     }

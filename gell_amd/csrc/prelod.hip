@@ -15,19 +15,14 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Clears the voxels the previous call wrote, and empties their x lines'
-// spans (every nonzero rate came from that call, so afterwards no line holds
-// one; lines of several listed voxels get the same value from each).
 __global__ __launch_bounds__(kBlock) void prelod_clear(const int32_t* __restrict__ list, int32_t m,
-                                                       long long* __restrict__ acc, float* __restrict__ rate,
-                                                       int2* __restrict__ xspan, int32_t N) {
+                                                       long long* __restrict__ acc, float* __restrict__ rate) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int32_t v = list[i];
     if (v >= 0) {
         acc[v] = 0;
         rate[v] = 0.f;
-        xspan[v / N] = make_int2(N, -1);
     }
 }
 
@@ -50,21 +45,16 @@ __global__ __launch_bounds__(kBlock) void prelod_accumulate(const gell_cell* __r
 
 __global__ __launch_bounds__(kBlock) void prelod_finalize(const int32_t* __restrict__ list, int32_t n,
                                                           const long long* __restrict__ acc,
-                                                          float* __restrict__ rate, int2* __restrict__ xspan,
-                                                          int32_t N) {
+                                                          float* __restrict__ rate) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t v = list[i];
-    if (v >= 0) {
-        rate[v] = from_fixed(acc[v]);
-        int* sp = reinterpret_cast<int*>(xspan + v / N);
-        atomicMin(sp, v % N);
-        atomicMax(sp + 1, v % N);
-    }
+    if (v >= 0) rate[v] = from_fixed(acc[v]);
 }
 
-// Spans from the map itself (after a caller may have written it): one wave
-// per x line, first / last nonzero voxel.
+// x-line spans of the uptake map: one wave per x line, first / last nonzero
+// voxel (one coalesced read of the map; per-cell atomics on the span of a
+// line contended too much).
 __global__ __launch_bounds__(kBlock) void xspan_scan(const float* __restrict__ rate, int2* __restrict__ xspan,
                                                      int32_t N) {
     const int64_t line = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
@@ -93,7 +83,7 @@ hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n
     const int32_t m = ctx->rate_list_n[s];
     if (m > 0)
         hipLaunchKernelGGL(prelod_clear, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ctx->rate_list[s], m,
-                           ctx->acc[s], ctx->rate[s], ctx->xspan[s], ctx->N);
+                           ctx->acc[s], ctx->rate[s]);
     ctx->rate_list_n[s] = 0;
     if (n > 0) {
         const gell_substrate& sb = ctx->P.sub[s];
@@ -101,9 +91,10 @@ hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n
         hipLaunchKernelGGL(prelod_accumulate, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->V, ctx->P.dx,
                            sb.uptake, sb.dead_factor, ctx->acc[s], ctx->rate_list[s]);
         hipLaunchKernelGGL(prelod_finalize, dim3(blocks), dim3(kBlock), 0, st, ctx->rate_list[s], n, ctx->acc[s],
-                           ctx->rate[s], ctx->xspan[s], ctx->N);
+                           ctx->rate[s]);
         ctx->rate_list_n[s] = n;
     }
+    ctx->xspan_valid[s] = false;  // rebuilt by the first substep that reads the spans
     return hipGetLastError();
 }
 

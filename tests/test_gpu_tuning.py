@@ -201,13 +201,14 @@ def test_substrate_streams_give_identical_results():
     assert np.array_equal(outs[0][2].view(np.uint8), outs[1][2].view(np.uint8))
 
 
-def test_rate_spans_kept_by_prelod_equal_spans_rebuilt_from_the_map(oracle):
+def test_rate_spans_rebuilt_after_prelod_equal_spans_rebuilt_every_substep(oracle):
     """The partitioned x sweep reads uptake rates only inside each x line's
-    span of voxels that may hold a nonzero rate. Spans kept by PreLOD over
-    several mechanics steps (cells move, the previous map is cleared) must
-    give the same bits as spans rebuilt from the map before every substep
-    (handing out the uptake-map pointer forces the rebuild), and the field
-    must stay within the 1e-5 relative-L2 bar of the oracle (the partitioned
+    span of voxels that may hold a nonzero rate, rebuilt from the map by the
+    first substep after each PreLOD. Over several mechanics steps (cells move,
+    the previous map is cleared) this must give the same bits as spans
+    rebuilt before every substep (handing out the uptake-map pointer forces
+    the rebuild), and the field must stay within the 1e-5 relative-L2 bar of
+    the oracle (a skipped nonzero rate would break it; the partitioned
     solve's rounding, under 1e-6 per substep, accumulates over 66 substeps)."""
     from gell_amd import spheroid
     from helpers import rel_l2
