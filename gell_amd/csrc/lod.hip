@@ -845,22 +845,12 @@ hipError_t launch_reg(float* p, const float* rate, const float* E, const float* 
     return hipGetLastError();
 }
 
-// Serial line kernels for an N that is not a multiple of the chunk; with
-// `plane` (N <= 128) the x and y sweeps run as one kernel per z plane.
+// Serial line kernels for an N that is not a multiple of the chunk (x sweep
+// staged whole through LDS, y and z in registers).
 template <int N>
 hipError_t launch_reg_flat(float* p, const float* rate, const float* E, const float* F, float side, float bnd,
-                           float dt, int spill, bool plane, hipStream_t st) {
+                           float dt, int spill, hipStream_t st) {
     const unsigned blocks = (unsigned)(((int64_t)N * N + 63) / 64);
-    if constexpr (N <= 128) {
-        if (plane) {
-            hipLaunchKernelGGL((lod_xy_plane<N>), dim3(N), dim3(128), 0, st, p, rate, E, F, side, bnd, dt);
-            if (spill > 0)
-                hipLaunchKernelGGL((lod_yz_reg<N, 2, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
-            else
-                hipLaunchKernelGGL((lod_yz_reg<N, 2, 0>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
-            return hipGetLastError();
-        }
-    }
     hipLaunchKernelGGL((lod_x_flat<N>), dim3(blocks), dim3(64), 0, st, p, rate, E, F, side, bnd, dt);
     if (spill > 0) {
         hipLaunchKernelGGL((lod_yz_reg<N, 1, 64>), dim3(blocks), dim3(64), 0, st, p, E, F, side, bnd);
@@ -1046,8 +1036,8 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
         case 100:
             if (xp > 1 || yzp > 1 || fused)
                 return launch_plane_part<100>(p, rate, E, F, ctx->ptab[s], side, bnd, dt, xp, yzp, spill, st);
-            return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, false, st);
-        case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, false, st);
+            return launch_reg_flat<100>(p, rate, E, F, side, bnd, dt, spill, st);
+        case 200: return launch_reg_flat<200>(p, rate, E, F, side, bnd, dt, spill, st);
         default: break;
     }
     const int64_t lines = (int64_t)N * N;

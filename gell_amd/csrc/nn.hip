@@ -398,22 +398,36 @@ __global__ void features(const float* __restrict__ p, const float* __restrict__ 
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (int64_t)X * X * X) return;
     const int x = (int)(v % X), y = (int)((v / X) % X), z = (int)(v / ((int64_t)X * X));
+    // the fine voxels come in x pairs: four 8-byte loads per field
+    float pv[8], rv[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int yi = 2 * y + (q & 1), zi = 2 * z + (q >> 1);
+        const int64_t o = ((int64_t)zi * N + yi) * N + 2 * x;
+        const float2 a = *reinterpret_cast<const float2*>(p + o);
+        const float2 r = *reinterpret_cast<const float2*>(rate + o);
+        pv[2 * q] = a.x;
+        pv[2 * q + 1] = a.y;
+        rv[2 * q] = r.x;
+        rv[2 * q + 1] = r.y;
+    }
     uint16_t h[24];
+#pragma unroll
     for (int d = 0; d < 8; ++d) {
-        const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
-        const int64_t o = ((int64_t)zi * N + yi) * N + xi;
-        const float q = p[o] * inv_bnd - 1.f;
+        const float q = pv[d] * inv_bnd - 1.f;
         h[d] = f2bf(q);
         h[8 + d] = f2bf(q - bf2f(h[d]));
-        h[16 + d] = f2bf(rate[o] * rate_scale);
+        h[16 + d] = f2bf(rv[d] * rate_scale);
     }
     uint4 pk[3];
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
         pk[q].x = (uint32_t)h[8 * q + 0] | ((uint32_t)h[8 * q + 1] << 16);
         pk[q].y = (uint32_t)h[8 * q + 2] | ((uint32_t)h[8 * q + 3] << 16);
         pk[q].z = (uint32_t)h[8 * q + 4] | ((uint32_t)h[8 * q + 5] << 16);
         pk[q].w = (uint32_t)h[8 * q + 6] | ((uint32_t)h[8 * q + 7] << 16);
     }
+#pragma unroll
     for (int q = 0; q < 3; ++q) reinterpret_cast<uint4*>(f + v * 24)[q] = pk[q];
 }
 

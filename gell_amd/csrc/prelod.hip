@@ -62,11 +62,25 @@ __global__ __launch_bounds__(kBlock) void xspan_scan(const float* __restrict__ r
     const int lane = lane_id();
     const float* r = rate + line * N;
     int lo = N, hi = -1;
-    for (int i = lane; i < N; i += 64)
-        if (r[i] != 0.f) {
-            lo = i < lo ? i : lo;
-            hi = i;
+    if ((N & 3) == 0) {  // lines are 16-byte aligned: one float4 per lane per step
+        for (int i4 = lane; i4 < N / 4; i4 += 64) {
+            const float4 q = reinterpret_cast<const float4*>(r)[i4];
+            const float qv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (qv[e] != 0.f) {
+                    const int i = 4 * i4 + e;
+                    lo = i < lo ? i : lo;
+                    hi = i;
+                }
         }
+    } else {
+        for (int i = lane; i < N; i += 64)
+            if (r[i] != 0.f) {
+                lo = i < lo ? i : lo;
+                hi = i;
+            }
+    }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
         const int olo = __shfl_xor(lo, d, 64), ohi = __shfl_xor(hi, d, 64);
