@@ -670,10 +670,8 @@ __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const fl
     constexpr int P = N / 64;
     constexpr int m = 64;  // x segment per wave = y rows per block
     constexpr int64_t NN = (int64_t)N * N;
-    constexpr int RowLd = N + 1;
     constexpr int XT = P * 64 * kTileLd;
-    constexpr int BT = 64 * RowLd;
-    __shared__ float smem[XT > BT ? XT : BT];
+    __shared__ float smem[XT];
     __shared__ float bt[P * 64], bb[P * 64];
     const int lane = threadIdx.x & 63;
     const int kx = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -681,13 +679,26 @@ __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const fl
     const int kz = (int)(L0 / N), yb = (int)((L0 % N) / 64);
     float v[m];
     x_lines_solve<N, P>(v, p, rate, xspan, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
-    // every wave is past its transpose-tile reads (part_solve synchronises)
+    // Column i = kx m + lane of the block comes from this wave's own 64 x 64
+    // segment (row r = lane r), so each wave transposes it through its own
+    // 64 x 33 tile in two halves of 32 columns. (Every wave is past its
+    // transpose-tile reads: part_solve synchronises.)
+    float* tw = smem + kx * 64 * kTileLd;
+    float col[m];
 #pragma unroll
-    for (int t = 0; t < m; ++t) smem[lane * RowLd + kx * m + t] = v[t];
-    __syncthreads();
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) tw[lane * kTileLd + t] = v[h * kChunk + t];
+        __syncthreads();
+        if ((lane >> 5) == h) {
+#pragma unroll
+            for (int r = 0; r < m; ++r) col[r] = tw[r * kTileLd + (lane & 31)];
+        }
+        __syncthreads();
+    }
     const int i = threadIdx.x;
 #pragma unroll
-    for (int t = 0; t < m; ++t) v[t] = smem[t * RowLd + i];
+    for (int t = 0; t < m; ++t) v[t] = col[t];
     if (yb == 0) v[0] = bnd;
     if (yb == P - 1) v[m - 1] = bnd;
     part_local<m>(v, tab + yb * m, tab + N + yb * m, side);
