@@ -523,7 +523,7 @@ def main() -> int:
                     help="kernel-variant knob for --only runs (e.g. lod_parts=8)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the library has no multi-GPU path",
               file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:

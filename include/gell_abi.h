@@ -102,7 +102,7 @@ typedef enum gell_status {
     GELL_E_ARG = -1,         /* bad argument (null pointer, size out of range) */
     GELL_E_HIP = -2,         /* a HIP runtime call failed */
     GELL_E_NOMEM = -3,       /* device allocation failed */
-    GELL_E_UNSUPPORTED = -4, /* parameter combination this build does not support */
+    GELL_E_UNSUPPORTED = -4, /* parameter combination the library does not support */
     GELL_E_NODEVICE = -5     /* no usable gfx950 device */
 } gell_status;
 
