@@ -110,6 +110,7 @@ void release(gell_ctx* c) {
     dfree(c->se_lin);
     dfree(c->se_list);
     dfree(c->mblk);
+    dfree(c->occ);
     dfree(c->n_out);
     for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) {
         dfree(c->field[s]);
@@ -220,6 +221,8 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->se_list, cap));
     c->nmblk = c->KS >= 512 ? c->KS / 512 : 1;
     A(dalloc(&c->mblk, (size_t)c->nmblk));
+    c->occ_words = c->KS >= 64 ? c->KS / 32 : 2;
+    A(dalloc(&c->occ, (size_t)c->occ_words));
     A(dalloc(&c->n_out, 1));
     for (int s = 0; s < P.n_substrates; ++s) {
         A(dalloc(&c->field[s], (size_t)c->V));
@@ -419,7 +422,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
     const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
-                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 5));
+                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 6));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     const hipError_t e = gell::join_substrates(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "tune");

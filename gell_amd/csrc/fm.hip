@@ -39,11 +39,15 @@ constexpr int kBlock = 256;
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void fm_keys(const gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
                                                   uint32_t* __restrict__ keys, const int32_t* __restrict__ clear_list,
-                                                  int32_t clear_n, int2* __restrict__ se, int32_t s_empty) {
+                                                  int32_t clear_n, int2* __restrict__ se, int32_t s_empty,
+                                                  uint32_t* __restrict__ occ) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < clear_n) {
         const int32_t v = clear_list[i];
-        if (v >= 0) se[v] = make_int2(s_empty, -1);
+        if (v >= 0) {
+            se[v] = make_int2(s_empty, -1);
+            occ[(uint32_t)v >> 5] = 0u;  // the whole word: each voxel set in it is on the list
+        }
     }
     if (i < n) {
         const float* pos = reinterpret_cast<const float*>(cells + i) + W_PX;
@@ -382,41 +386,71 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
                                                     gell_cell* __restrict__ sorted, float4* __restrict__ nb,
                                                     int2* __restrict__ se, int32_t* __restrict__ se_list,
                                                     const int32_t* __restrict__ skeys, int2* __restrict__ blk,
-                                                    int32_t* __restrict__ n_out) {
+                                                    int32_t* __restrict__ n_out, uint32_t* __restrict__ occ) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (blk) {
-        // Boundaries of the aligned 8^3 Morton blocks in sorted order (keys are
-        // sorted, so each boundary has exactly one writer); the outside
-        // cells (key -1) sort first and their count goes to *n_out.
-        const auto block_of = [&](int64_t q) { return skeys[q] >= 0 ? skeys[q] >> 9 : -1; };
-        const int32_t b = block_of(i);
-        const int32_t bp = i > 0 ? block_of(i - 1) : -2, bn = i + 1 < n ? block_of(i + 1) : -2;
-        if (b >= 0) {
-            if (bp != b) blk[b].x = (int32_t)i;
-            if (bn != b) blk[b].y = (int32_t)i;
-        } else if (bn != b) {
-            *n_out = (int32_t)i + 1;
+    // Voxel (Morton code; -1 outside, INT32_MAX past n: non-decreasing along
+    // the wave), occupancy word (code / 32) and bit, and the slot if live.
+    int32_t kv = INT32_MAX, ow = INT32_MAX;
+    uint32_t obit = 0u;
+    int32_t lmin = INT32_MAX, lmax = -1;
+    if (i < n) {
+        if (blk) {
+            // Boundaries of the aligned 8^3 Morton blocks in sorted order (keys
+            // are sorted, so each boundary has exactly one writer); the outside
+            // cells (key -1) sort first and their count goes to *n_out.
+            const auto block_of = [&](int64_t q) { return skeys[q] >= 0 ? skeys[q] >> 9 : -1; };
+            const int32_t b = block_of(i);
+            const int32_t bp = i > 0 ? block_of(i - 1) : -2, bn = i + 1 < n ? block_of(i + 1) : -2;
+            if (b >= 0) {
+                if (bp != b) blk[b].x = (int32_t)i;
+                if (bn != b) blk[b].y = (int32_t)i;
+            } else if (bn != b) {
+                *n_out = (int32_t)i + 1;
+            }
         }
+        CellRegs c = load_cell(cells, order[i]);
+        const float px = cf(c, W_PX), py = cf(c, W_PY), pz = cf(c, W_PZ);
+        const int32_t a = trunc_index(px / dx), b = trunc_index(py / dx), k = trunc_index(pz / dx);
+        // updateCellMeshIdx's record writes (FMkernel.cuh:26-35).
+        const bool in = inside(a, b, k, N);
+        if (!in)
+            cset_sign(c, false);
+        else
+            c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
+        store_cell(sorted, i, c);
+        const bool live = csign(c);
+        const uint32_t code = in ? morton3(a, b, k) : 0u;
+        const int32_t v = live ? (int32_t)code : -1;
+        se_list[i] = v;
+        nb[i] = make_float4(px, py, pz, live ? cf(c, W_R) : __uint_as_float(kDeadTag));
+        kv = in ? (int32_t)code : -1;
+        ow = in ? (int32_t)(code >> 5) : -1;
+        obit = live ? 1u << (code & 31u) : 0u;
+        lmin = live ? (int32_t)i : INT32_MAX;
+        lmax = live ? (int32_t)i : -1;
     }
-    CellRegs c = load_cell(cells, order[i]);
-    const float px = cf(c, W_PX), py = cf(c, W_PY), pz = cf(c, W_PZ);
-    const int32_t a = trunc_index(px / dx), b = trunc_index(py / dx), k = trunc_index(pz / dx);
-    // updateCellMeshIdx's record writes (FMkernel.cuh:26-35).
-    if (!inside(a, b, k, N))
-        cset_sign(c, false);
-    else
-        c.w[W_MESH] = (uint32_t)lin(a, b, k, N);
-    store_cell(sorted, i, c);
-    const bool live = csign(c);
-    const int32_t v = live ? (int32_t)morton3(a, b, k) : -1;
-    se_list[i] = v;
-    nb[i] = make_float4(px, py, pz, live ? cf(c, W_R) : __uint_as_float(kDeadTag));
-    if (live) {
-        // Mech_Mesh_se_Get, FMkernel.cuh:45-53: min/max of the live slots.
-        atomicMin(&se[v].x, (int)i);
-        atomicMax(&se[v].y, (int)i);
+    // Slots are in key order, so the lanes of one voxel, and those of one
+    // occupancy word, are adjacent runs. Segmented scans give the first lane
+    // of each run the run's smallest live slot and OR of bits, and the last
+    // lane its largest live slot; only those lanes issue atomics (runs cut by
+    // a wave boundary combine through the atomics).
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d *= 2) {
+        const uint32_t b2 = __shfl_down(obit, d);
+        const int32_t w2 = __shfl_down(ow, d), k2 = __shfl_down(kv, d), m2 = __shfl_down(lmin, d);
+        const int32_t ku = __shfl_up(kv, d), mu = __shfl_up(lmax, d);
+        if (lane + d < 64) {
+            if (w2 == ow) obit |= b2;
+            if (k2 == kv) lmin = m2 < lmin ? m2 : lmin;
+        }
+        if (lane >= d && ku == kv) lmax = mu > lmax ? mu : lmax;
     }
+    const int32_t wprev = __shfl_up(ow, 1), kprev = __shfl_up(kv, 1), knext = __shfl_down(kv, 1);
+    // Mech_Mesh_se_Get, FMkernel.cuh:45-53: min/max of the live slots.
+    if (lmin != INT32_MAX && (lane == 0 || kprev != kv)) atomicMin(&se[kv].x, lmin);
+    if (lmax >= 0 && (lane == 63 || knext != kv)) atomicMax(&se[kv].y, lmax);
+    if (obit != 0u && (lane == 0 || wprev != ow)) atomicOr(&occ[ow], obit);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +660,87 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
     }
 }
 
+// Occupancy-mask variant: the stencil voxels that hold a live cell are read
+// first from the occupancy bits (one bit per Morton code, so a 64-bit word is
+// an aligned 4 x 4 x 4 block; the stencil touches at most 8 such words), and
+// only the non-empty voxels' ranges are loaded (at ~0.1 cells per voxel,
+// about 4 of 27). Each range is loaded while the previous voxel's candidates
+// are walked, and the load one candidate ahead runs on into the next voxel.
+// Visiting order as in the other variants: x outer, z inner, slots ascending.
+__device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, const int2* __restrict__ se,
+                                               const uint32_t* __restrict__ occ, int32_t N, float dx, float p1x,
+                                               float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
+    const __amdgpu_buffer_rsrc_t ro =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(occ), (short)0, 0x7FFFFFFF, 0x00020000);
+    // 4^3 blocks of the clamped stencil [p - 1, p + 1] per axis: b0 and b1
+    const int32_t bx0 = (px > 0 ? px - 1 : 0) >> 2, bx1 = (px + 1 < N ? px + 1 : N - 1) >> 2;
+    const int32_t by0 = (py > 0 ? py - 1 : 0) >> 2, by1 = (py + 1 < N ? py + 1 : N - 1) >> 2;
+    const int32_t bz0 = (pz > 0 ? pz - 1 : 0) >> 2, bz1 = (pz + 1 < N ? pz + 1 : N - 1) >> 2;
+    uint64_t W[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t wi = morton3(a ? bx1 : bx0, b ? by1 : by0, c ? bz1 : bz0);
+                const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * 8u), 0, 0));
+                W[a][b][c] = ((uint64_t)w.y << 32) | w.x;
+            }
+    // bit q = 9 xo + 3 yo + zo of the stencil voxel (px - 1 + xo, py - 1 + yo, pz - 1 + zo);
+    // inside a block, voxel (x, y, z) is bit morton3(x & 3, y & 3, z & 3)
+    const auto low = [](int32_t v, int sh) { return (uint32_t)(((v & 1) << sh) | ((v & 2) << (sh + 2))); };
+    uint32_t mask = 0;
+#pragma unroll
+    for (int xo = 0; xo < 3; ++xo) {
+        const int32_t x = px - 1 + xo;
+        if ((uint32_t)x >= (uint32_t)N) continue;
+        const int a = (x >> 2) != bx0;
+#pragma unroll
+        for (int yo = 0; yo < 3; ++yo) {
+            const int32_t y = py - 1 + yo;
+            if ((uint32_t)y >= (uint32_t)N) continue;
+            const int b = (y >> 2) != by0;
+            const uint64_t w0 = a ? (b ? W[1][1][0] : W[1][0][0]) : (b ? W[0][1][0] : W[0][0][0]);
+            const uint64_t w1 = a ? (b ? W[1][1][1] : W[1][0][1]) : (b ? W[0][1][1] : W[0][0][1]);
+            const uint32_t mxy = low(x, 0) | low(y, 1);
+#pragma unroll
+            for (int zo = 0; zo < 3; ++zo) {
+                const int32_t z = pz - 1 + zo;
+                if ((uint32_t)z >= (uint32_t)N) continue;
+                const uint64_t w = ((z >> 2) != bz0) ? w1 : w0;
+                mask |= (uint32_t)((w >> (mxy | low(z, 2))) & 1u) << (9 * xo + 3 * yo + zo);
+            }
+        }
+    }
+    if (mask == 0) return;
+    const auto range_of = [&](uint32_t q) {
+        return se[morton3(px - 1 + (int)(q / 9), py - 1 + (int)((q / 3) % 3), pz - 1 + (int)(q % 3))];
+    };
+    int2 cur = range_of((uint32_t)__builtin_ctz(mask));
+    mask &= mask - 1;
+    int j = cur.x;
+    float4 o = load_nb(rs, j);
+    while (true) {
+        const bool more = mask != 0;
+        const int2 nxt = more ? range_of((uint32_t)__builtin_ctz(mask)) : make_int2(0, -1);
+        mask &= mask - 1;
+        const int e = cur.y;
+        for (; j <= e; ++j) {
+            // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
+            const float4 ahead = load_nb(rs, j < e ? j + 1 : (more ? nxt.x : j));
+            force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            o = ahead;
+        }
+        if (!more) break;
+        cur = nxt;
+        j = cur.x;
+    }
+}
+
 // movementkernel, FMkernel.cuh:130-158: AB2 step of a live cell; outside the
 // domain the cell is flagged dead and keeps its position.
 __device__ __forceinline__ void move_cell(CellRegs& c, float fx, float fy, float fz, int32_t N, float dx, float dt) {
@@ -656,13 +771,13 @@ __device__ __forceinline__ void move_cell(CellRegs& c, float fx, float fy, float
 }
 
 // VARIANT 0: stencil voxels in turn; 1: their ranges loaded up front;
-// 2: flattened per-lane candidate walk; 3: grouped candidate loads. All four
-// give the same bits.
+// 2: flattened per-lane candidate walk; 3: grouped candidate loads;
+// 6: non-empty voxels from the occupancy map. All give the same bits.
 template <int VARIANT>
 __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
                                                         const float4* __restrict__ nb, const int2* __restrict__ se,
                                                         gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
-                                                        float dt) {
+                                                        float dt, const uint32_t* __restrict__ occ) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in_range = i < n;
     // The cell's own (x, y, z, r) and liveness come from its 16-byte entry of
@@ -677,6 +792,8 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
                         live ? me.w : 0.f, fx, fy, fz);
     } else if (VARIANT == 3) {
         if (live) cell_force_grouped(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+    } else if (VARIANT == 6) {
+        if (live) cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     } else if (live) {
         cell_force<VARIANT == 1>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     }
@@ -828,6 +945,11 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_blocks(const gell_cell* 
     }
 }
 
+__global__ void occ_clear(uint32_t* occ, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) occ[i] = 0u;
+}
+
 __global__ void se_fill(int2* se, int64_t V, int32_t s_empty) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V) se[v] = make_int2(s_empty, -1);
@@ -859,6 +981,8 @@ SortPlan plan_sort(int key_bits) {
 hipError_t fm_init_ranges(gell_ctx* ctx) {
     const unsigned blocks = (unsigned)((ctx->KS + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(se_fill, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, ctx->KS, ctx->P.capacity + 1);
+    hipLaunchKernelGGL(occ_clear, dim3((unsigned)((ctx->occ_words + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, ctx->occ, ctx->occ_words);
     ctx->se_list_n = 0;
     return hipGetLastError();
 }
@@ -873,7 +997,8 @@ hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     const int32_t m = n > ctx->se_list_n ? n : ctx->se_list_n;
     if (m == 0) return hipSuccess;
     hipLaunchKernelGGL(fm_keys, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells, n, ctx->N,
-                       ctx->P.dx, ctx->keys[0], ctx->se_list, ctx->se_list_n, ctx->se, ctx->P.capacity + 1);
+                       ctx->P.dx, ctx->keys[0], ctx->se_list, ctx->se_list_n, ctx->se, ctx->P.capacity + 1,
+                       ctx->occ);
     ctx->se_list_n = 0;
     return hipGetLastError();
 }
@@ -907,7 +1032,7 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
                            ctx->stream, ctx->mblk, ctx->nmblk, ctx->n_out);
     hipLaunchKernelGGL(fm_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
                        ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ctx->se, ctx->se_list,
-                       ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out);
+                       ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out, ctx->occ);
     ctx->se_list_n = n;
     return hipGetLastError();
 }
@@ -926,18 +1051,21 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     if (v == 4)
         hipLaunchKernelGGL(fm_force_move_staged, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
                            cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+    else if (v == 6)
+        hipLaunchKernelGGL(fm_force_move<6>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
     else if (v == 3)
         hipLaunchKernelGGL(fm_force_move<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
     else if (v == 2)
         hipLaunchKernelGGL(fm_force_move<2>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
     else if (v == 1)
         hipLaunchKernelGGL(fm_force_move<1>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
     else
         hipLaunchKernelGGL(fm_force_move<0>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
     return hipGetLastError();
 }
 

@@ -34,6 +34,11 @@ struct gell_ctx {
     int2* mblk = nullptr;          // per aligned 8^3 Morton block: first / last sorted slot (force variant 5)
     int64_t nmblk = 0;             // blocks: max(1, KS / 512)
     int32_t* n_out = nullptr;      // device scalar: cells outside the domain (they sort first)
+    // One bit per Morton code (bit v & 31 of word v / 32): set when voxel v
+    // holds a live cell after the last fm_update (force variant 6).
+    // max(KS / 32, 2) words.
+    uint32_t* occ = nullptr;
+    int64_t occ_words = 0;
     int32_t se_list_n = 0;
 
     // Diffusion, per substrate.
@@ -64,7 +69,7 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 1, 4, 4, 1, 1};
+    int tune[GELL_N_KNOBS] = {1, 6, 4, 4, 1, 1};
 
     // Every substrate runs its LOD substeps on a stream of its own when
     // GELL_KNOB_SUBSTRATE_STREAMS is set: forked from the context stream after

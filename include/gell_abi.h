@@ -181,8 +181,13 @@ int gell_sync(gell_ctx* ctx);
  *                          5 = one workgroup per aligned 8x8x8 Morton block:
  *                              the ranges of its 10^3 halo voxels and all
  *                              their candidates loaded into LDS once, each
- *                              cell's walk read from there.
- *                              Default 1.
+ *                              cell's walk read from there,
+ *                          6 = the stencil voxels that hold a live cell read
+ *                              first from a one-bit-per-voxel occupancy map
+ *                              (Morton-indexed: at most eight 8-byte reads),
+ *                              only their ranges loaded, each while the
+ *                              previous voxel is walked.
+ *                              Default 6.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4, 8 or 16 = each line split into
