@@ -660,19 +660,12 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
     }
 }
 
-// Occupancy-mask variant: the stencil voxels that hold a live cell are read
-// first from the occupancy bits (one bit per Morton code, so a 64-bit word is
-// an aligned 4 x 4 x 4 block; the stencil touches at most 8 such words), and
-// only the non-empty voxels' ranges are loaded (at ~0.1 cells per voxel,
-// about 4 of 27). Each range is loaded while the previous voxel's candidates
-// are walked, and the load one candidate ahead runs on into the next voxel.
-// Visiting order as in the other variants: x outer, z inner, slots ascending.
-__device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, const int2* __restrict__ se,
-                                               const uint32_t* __restrict__ occ, int32_t N, float dx, float p1x,
-                                               float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
-    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
-    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
-    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
+// Stencil voxels that hold a live cell, as a 27-bit mask (bit q = 9 xo +
+// 3 yo + zo for voxel (px - 1 + xo, py - 1 + yo, pz - 1 + zo)), read from the
+// occupancy bits: one bit per Morton code, so a 64-bit word is an aligned
+// 4 x 4 x 4 block and the stencil touches at most 8 such words.
+__device__ __forceinline__ uint32_t occ_mask(const uint32_t* __restrict__ occ, int32_t N, int32_t px, int32_t py,
+                                             int32_t pz) {
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(occ), (short)0, 0x7FFFFFFF, 0x00020000);
     // 4^3 blocks of the clamped stencil [p - 1, p + 1] per axis: b0 and b1
@@ -690,7 +683,6 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
                 const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * 8u), 0, 0));
                 W[a][b][c] = ((uint64_t)w.y << 32) | w.x;
             }
-    // bit q = 9 xo + 3 yo + zo of the stencil voxel (px - 1 + xo, py - 1 + yo, pz - 1 + zo);
     // inside a block, voxel (x, y, z) is bit morton3(x & 3, y & 3, z & 3)
     const auto low = [](int32_t v, int sh) { return (uint32_t)(((v & 1) << sh) | ((v & 2) << (sh + 2))); };
     uint32_t mask = 0;
@@ -716,6 +708,21 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
             }
         }
     }
+    return mask;
+}
+
+// Occupancy-mask variant: only the ranges of the stencil voxels that hold a
+// live cell are loaded (at ~0.1 cells per voxel, about 4 of 27). Each range is
+// loaded while the previous voxel's candidates are walked, and the load one
+// candidate ahead runs on into the next voxel. Visiting order as in the other
+// variants: x outer, z inner, slots ascending.
+__device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, const int2* __restrict__ se,
+                                               const uint32_t* __restrict__ occ, int32_t N, float dx, float p1x,
+                                               float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
+    uint32_t mask = occ_mask(occ, N, px, py, pz);
     if (mask == 0) return;
     const auto range_of = [&](uint32_t q) {
         return se[morton3(px - 1 + (int)(q / 9), py - 1 + (int)((q / 3) % 3), pz - 1 + (int)(q % 3))];
@@ -772,7 +779,7 @@ __device__ __forceinline__ void move_cell(CellRegs& c, float fx, float fy, float
 
 // VARIANT 0: stencil voxels in turn; 1: their ranges loaded up front;
 // 2: flattened per-lane candidate walk; 3: grouped candidate loads;
-// 6: non-empty voxels from the occupancy map. All give the same bits.
+// 6: non-empty voxels from the occupancy bitmap. All give the same bits.
 template <int VARIANT>
 __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
                                                         const float4* __restrict__ nb, const int2* __restrict__ se,
@@ -794,6 +801,7 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
         if (live) cell_force_grouped(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     } else if (VARIANT == 6) {
         if (live) cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+
     } else if (live) {
         cell_force<VARIANT == 1>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     }
