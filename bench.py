@@ -244,10 +244,17 @@ def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | No
     elif what == "copy":
         nbytes = 256 << 20
         a, b = rt.DeviceBuffer(nbytes), rt.DeviceBuffer(nbytes)
-        for _ in range(warmup + steps):
+        for k in range(warmup + steps):
+            if k == warmup:
+                ctx.lib.gell_device_sync()
+                t0 = time.perf_counter()
             ctx.lib.gell_stream_copy(b.ptr, a.ptr, nbytes, None)
         ctx.lib.gell_device_sync()
+        dt = (time.perf_counter() - t0) / max(steps, 1)
         out["bytes_per_launch_each_way"] = nbytes
+        # host clock around back-to-back launches: the achievable streaming rate
+        out["copy_us"] = dt * 1e6
+        out["copy_GBps"] = 2 * nbytes / dt / 1e9
         a.free()
         b.free()
     else:

@@ -274,6 +274,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint32_t* __
 constexpr int kSmallThreads = 1024, kSmallItems = 16, kSmallWaves = kSmallThreads / 64;
 constexpr int kSmallPer = 64 * kSmallItems;
 constexpr int kSmallMax = kSmallThreads * kSmallItems;
+// Digits of up to 11 bits: per-wave counts of 2048 bins take 128 KB of LDS,
+// and the keys of grids up to 128^3 (22 bits) sort in two passes.
+constexpr int kSmallBits = 11, kSmallBins = 1 << kSmallBits;
 
 __device__ __forceinline__ uint64_t match_digit_rt(uint32_t d, uint64_t valid, int bits) {
     uint64_t peers = valid;
@@ -290,8 +293,8 @@ __global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __re
                                                                   int32_t* __restrict__ order_out,
                                                                   int32_t* __restrict__ signed_out, int32_t n,
                                                                   SortPlan plan) {
-    __shared__ int cnt[kSmallWaves][512];
-    __shared__ int dbase[512];
+    __shared__ int cnt[kSmallWaves][kSmallBins];
+    __shared__ int dbase[kSmallBins];
     __shared__ int lds16[kSmallWaves];
     const int tid = threadIdx.x, lane = lane_id(), w = tid / 64;
     uint32_t* kin = k0;
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __re
         const int bits = plan.bits[q], shift = plan.shift[q], bins = 1 << bits;
         const bool last = q == plan.passes - 1;
         int32_t* vout = last ? order_out : vbuf[1 - cur];
-        for (int b = tid; b < kSmallWaves * 512; b += kSmallThreads) (&cnt[0][0])[b] = 0;
+        for (int b = tid; b < kSmallWaves * kSmallBins; b += kSmallThreads) (&cnt[0][0])[b] = 0;
         __syncthreads();
         uint32_t k[kSmallItems];
         int32_t val[kSmallItems];
@@ -324,22 +327,30 @@ __global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __re
             if (valid && lane == 63 - __clzll(peers)) cnt[w][d] = before + __popcll(peers);
         }
         __syncthreads();
-        // per digit: offsets of the waves (in place) and the digit's total
-        int acc = 0;
-        if (tid < bins) {
-            for (int ww = 0; ww < kSmallWaves; ++ww) {
-                const int c = cnt[ww][tid];
-                cnt[ww][tid] = acc;
-                acc += c;
+        // per digit: offsets of the waves (in place) and the digit's total;
+        // thread t owns digits 2t and 2t + 1
+        int acc[2] = {0, 0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int dg = 2 * tid + h;
+            if (dg < bins) {
+#pragma unroll 4
+                for (int ww = 0; ww < kSmallWaves; ++ww) {
+                    const int c = cnt[ww][dg];
+                    cnt[ww][dg] = acc[h];
+                    acc[h] += c;
+                }
             }
         }
-        // exclusive scan of the totals over digits (thread = digit)
-        const int inc = wave_inclusive_scan(acc);
+        // exclusive scan of the totals over digits
+        const int pair = acc[0] + acc[1];
+        const int inc = wave_inclusive_scan(pair);
         if (lane == 63) lds16[w] = inc;
         __syncthreads();
         int off = 0;
         for (int ww = 0; ww < w; ++ww) off += lds16[ww];
-        if (tid < bins) dbase[tid] = off + inc - acc;
+        if (2 * tid < bins) dbase[2 * tid] = off + inc - pair;
+        if (2 * tid + 1 < bins) dbase[2 * tid + 1] = off + inc - pair + acc[0];
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < kSmallItems; ++it) {
@@ -801,7 +812,6 @@ __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restr
         if (live) cell_force_grouped(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     } else if (VARIANT == 6) {
         if (live) cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
-
     } else if (live) {
         cell_force<VARIANT == 1>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     }
@@ -974,9 +984,9 @@ __global__ void se_to_linear(const int2* __restrict__ se, int2* __restrict__ out
 
 }  // namespace
 
-SortPlan plan_sort(int key_bits) {
+SortPlan plan_sort(int key_bits, int max_bits) {
     SortPlan p{};
-    p.passes = key_bits <= 9 ? 1 : (key_bits + 8) / 9;
+    p.passes = key_bits <= max_bits ? 1 : (key_bits + max_bits - 1) / max_bits;
     int shift = 0;
     for (int q = 0; q < p.passes; ++q) {
         p.bits[q] = key_bits / p.passes + (q < key_bits % p.passes ? 1 : 0);
@@ -1013,12 +1023,14 @@ hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 
 hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
     if (n == 0) return hipSuccess;
-    const SortPlan plan = plan_sort(3 * ceil_log2(ctx->N) + 1);
+    const int key_bits = 3 * ceil_log2(ctx->N) + 1;
     if (n <= kSmallMax) {
         hipLaunchKernelGGL(radix_sort_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ctx->keys[0], ctx->vals[0],
-                           ctx->keys[1], ctx->vals[1], ctx->out_order, ctx->out_keys, n, plan);
+                           ctx->keys[1], ctx->vals[1], ctx->out_order, ctx->out_keys, n,
+                           plan_sort(key_bits, kSmallBits));
         return hipGetLastError();
     }
+    const SortPlan plan = plan_sort(key_bits, 9);
     int cur = 0;
     for (int q = 0; q < plan.passes; ++q) {
         const bool last = q == plan.passes - 1;

@@ -134,13 +134,14 @@ hipError_t birth_finish(gell_ctx* ctx, int32_t n, int32_t* n_out);
 hipError_t death_begin(gell_ctx* ctx, gell_cell* cells, int32_t n);
 hipError_t death_finish(gell_ctx* ctx, int32_t n, int32_t* n_out);
 
-// Radix-sort planning: number of passes and bits per pass for a key width.
+// Radix-sort planning: number of passes and bits per pass (at most
+// max_bits, as even as possible) for a key width.
 struct SortPlan {
     int passes;
     int bits[8];
     int shift[8];
 };
-SortPlan plan_sort(int key_bits);
+SortPlan plan_sort(int key_bits, int max_bits);
 
 inline int ceil_log2(int v) {
     int b = 0;
