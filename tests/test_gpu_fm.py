@@ -45,6 +45,28 @@ def test_fm_steps_match_oracle(N, n, dense, oracle):
     st.close()
 
 
+@pytest.mark.parametrize("N", [13, 30])
+def test_fm_faces_of_odd_grids_match_oracle(N, oracle):
+    """Grid sides that are not multiples of 4 (the occupancy words cover
+    aligned 4^3 blocks, which then stick out of the grid), with a third of the
+    cells in the first or last voxel layer of some axis, where the stencil
+    is clipped."""
+    n = 3000
+    log(f"[fm] faces N={N}")
+    P = params(N, capacity=n)
+    cells = mixed_cells(N, 10.0, n, P.capacity, seed=N, dense=True)
+    rng = np.random.default_rng(N)
+    k = n // 3
+    axis, high = rng.integers(0, 3, k), rng.integers(0, 2, k) == 1
+    layer = np.where(high, rng.uniform((N - 1) * 10.0, N * 10.0 - 0.01, k), rng.uniform(0.0, 10.0, k))
+    cells["pos"][np.arange(k), axis] = layer.astype(np.float32)
+    st = GpuState(P, cells)
+    state = cells
+    for step in range(3):
+        state = _check_step(st, state, n, oracle, f"N={N} faces step {step}")
+    st.close()
+
+
 def test_fm_chained_steps_stay_identical(oracle):
     # GPU state is never re-synchronised: ten steps back to back.
     N, n = 48, 5000

@@ -28,11 +28,12 @@ def test_lod_variants_identical(N):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000)])
+@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000), (13, 3000)])
 def test_fm_variants_identical(N, n):
     """All force-kernel variants give the same bits; an odd count exercises
     the staged variant's 8-byte record tail, N = 100 Morton blocks that
-    stick out of the grid (block variant), and the mixed cells include dead
+    stick out of the grid (block variant), N = 13 occupancy words (aligned
+    4^3 blocks) clipped by the grid's faces, and the mixed cells include dead
     and outside ones."""
     P = params(N, capacity=n)
     cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
