@@ -467,8 +467,6 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
 // ---------------------------------------------------------------------------
 // Force + movement
 // ---------------------------------------------------------------------------
-// One candidate pair of forcekernel (FMkernel.cuh:86-109): returns false
-// when the pair is skipped; thres carries over between calls.
 // One 16-byte neighbour entry. The raw buffer load cannot be split: a plain
 // float4 load followed by the dead-tag test on .w was compiled into a 4-byte
 // load, a wait, and a conditional 12-byte load -- two memory round trips per
@@ -480,16 +478,36 @@ __device__ __forceinline__ float4 load_nb(__amdgpu_buffer_rsrc_t rs, int32_t j) 
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, j * 16, 0, 0));
 }
 
-__device__ __forceinline__ void force_pair(float4 o, float p1x, float p1y, float p1z, float r1, float& thres,
+// thres of forcekernel (Tumor_CC_Max_Dist at first, then 1.25 (r1 + r2) of
+// the last accepted pair; FMkernel.cuh:72, 96) with a squared-distance bound
+// ss_max = thres^2 (1 + 2^-16), rounded: for ss >= ss_max the pair test's
+// distance, sqrtf of ss + 0.0001 rounded to float, is at least
+// thres (1 + 7.5e-6) before its final rounding and so >= thres after it. Far
+// candidates are rejected on ss alone, without the square root; every
+// candidate the bound lets through takes the exact test.
+struct Cutoff {
+    float thres, ss_max;
+    __device__ Cutoff() { set(25.f); }
+    __device__ void set(float t) {
+        thres = t;
+        ss_max = t * t * 1.0000153f;
+    }
+};
+
+// One candidate pair of forcekernel (FMkernel.cuh:86-109); the cutoff carries
+// over between calls.
+__device__ __forceinline__ void force_pair(float4 o, float p1x, float p1y, float p1z, float r1, Cutoff& cut,
                                            float& fx, float& fy, float& fz) {
     if (__float_as_uint(o.w) == kDeadTag) return;
     const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
     const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+    if (ss >= cut.ss_max) return;
     const float distance = sqrtf((float)((double)ss + 0.0001));
-    if (distance >= thres) return;
+    if (distance >= cut.thres) return;
     const float rsum = o.w + r1;
     const float R = rsum > 0.01f ? rsum : 0.01f;
-    thres = (float)((double)R * 1.25);
+    cut.set((float)((double)R * 1.25));
+    const float thres = cut.thres;
     if (distance >= thres) return;
     const float qr = 1.f - distance / R;
     const float qa = 1.f - distance / thres;
@@ -510,7 +528,7 @@ __device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb
                                                    int32_t N, float dx, float p1x, float p1y, float p1z, float r1,
                                                    float& fx, float& fy, float& fz) {
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
-    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     int2 rr[27];
 #pragma unroll
@@ -531,13 +549,13 @@ __device__ __forceinline__ void cell_force_grouped(const float4* __restrict__ nb
         for (int t = 0; t < 9; ++t) {
             const int2 r = rr[g * 9 + t];
             if (r.y < 0) continue;
-            force_pair(c0[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
-            if (r.y > r.x) force_pair(c1[t], p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            force_pair(c0[t], p1x, p1y, p1z, r1, cut, fx, fy, fz);
+            if (r.y > r.x) force_pair(c1[t], p1x, p1y, p1z, r1, cut, fx, fy, fz);
             if (r.y > r.x + 1) {
                 float4 o = load_nb(rs, r.x + 2);
                 for (int j = r.x + 2; j <= r.y; ++j) {
                     const float4 nxt = load_nb(rs, j < r.y ? j + 1 : j);  // one candidate ahead
-                    force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+                    force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
                     o = nxt;
                 }
             }
@@ -555,7 +573,7 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
                                            float dx, float p1x, float p1y, float p1z, float r1, float& fx, float& fy,
                                            float& fz) {
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
-    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     if (PREFETCH) {
         // last[q]: last candidate of stencil voxel q (-1: empty); nf[q]: first
@@ -588,7 +606,7 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
             for (; j <= e; ++j) {
                 const int ni = j < e ? j + 1 : (nf[q] >= 0 ? nf[q] : j);
                 const float4 nxt = load_nb(rs, ni);
-                force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+                force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
                 o = nxt;
             }
             j = nf[q];
@@ -605,7 +623,7 @@ __device__ __forceinline__ void cell_force(const float4* __restrict__ nb, const 
         float4 o = load_nb(rs, r.x);
         for (int j = r.x; j <= r.y; ++j) {
             const float4 nxt = load_nb(rs, j < r.y ? j + 1 : j);
-            force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
             o = nxt;
         }
     }
@@ -633,7 +651,7 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
 #pragma unroll
         for (int q = 0; q < 27; ++q) mask |= (rr[q].y >= 0 ? 1u : 0u) << q;
     }
-    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     int j = 0, je = -1;
     while (true) {
@@ -650,72 +668,84 @@ __device__ __forceinline__ void cell_force_flat(const float4* __restrict__ nb, c
         if (!active) continue;
         const float4 o = load_nb(rs, j);
         ++j;
-        if (__float_as_uint(o.w) == kDeadTag) continue;
-        // FMkernel.cuh:89-109
-        const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
-        const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
-        const float distance = sqrtf((float)((double)ss + 0.0001));
-        if (distance >= thres) continue;
-        const float rsum = o.w + r1;
-        const float R = rsum > 0.01f ? rsum : 0.01f;
-        thres = (float)((double)R * 1.25);
-        if (distance >= thres) continue;
-        const float qr = 1.f - distance / R;
-        const float qa = 1.f - distance / thres;
-        const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
-        const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
-        const float cc = coefr - coefa;
-        fx += ddx * cc;
-        fy += ddy * cc;
-        fz += ddz * cc;
+        force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);  // FMkernel.cuh:89-109
     }
+}
+
+// Morton codes of the stencil's coordinates, per axis in dilated form (the
+// axis's bits spread to every third bit): code(x, y, z) = cx | cy | cz. The
+// neighbours' codes come from dilated increments and decrements of the
+// cell's own (FMkernel.cuh:22-36 for the code), not from three spreads each.
+struct StencilCodes {
+    uint32_t cx[3], cy[3], cz[3];
+};
+
+template <uint32_t M>
+__device__ __forceinline__ uint32_t dilated_inc(uint32_t s) {
+    return ((s | ~M) + (M & (0u - M))) & M;
+}
+template <uint32_t M>
+__device__ __forceinline__ uint32_t dilated_dec(uint32_t s) {
+    return (s - (M & (0u - M))) & M;
+}
+
+__device__ __forceinline__ StencilCodes stencil_codes(int32_t px, int32_t py, int32_t pz) {
+    constexpr uint32_t MX = 0x09249249u, MY = MX << 1, MZ = MX << 2;
+    const uint32_t sx = spread3((uint32_t)px), sy = spread3((uint32_t)py) << 1, sz = spread3((uint32_t)pz) << 2;
+    StencilCodes c;
+    c.cx[0] = dilated_dec<MX>(sx);
+    c.cx[1] = sx;
+    c.cx[2] = dilated_inc<MX>(sx);
+    c.cy[0] = dilated_dec<MY>(sy);
+    c.cy[1] = sy;
+    c.cy[2] = dilated_inc<MY>(sy);
+    c.cz[0] = dilated_dec<MZ>(sz);
+    c.cz[1] = sz;
+    c.cz[2] = dilated_inc<MZ>(sz);
+    return c;
 }
 
 // Stencil voxels that hold a live cell, as a 27-bit mask (bit q = 9 xo +
 // 3 yo + zo for voxel (px - 1 + xo, py - 1 + yo, pz - 1 + zo)), read from the
-// occupancy bits: one bit per Morton code, so a 64-bit word is an aligned
-// 4 x 4 x 4 block and the stencil touches at most 8 such words.
+// occupancy bits: one bit per Morton code, so a 64-bit word (code / 64) is an
+// aligned 4 x 4 x 4 block and the stencil touches at most 8 such words.
 __device__ __forceinline__ uint32_t occ_mask(const uint32_t* __restrict__ occ, int32_t N, int32_t px, int32_t py,
-                                             int32_t pz) {
+                                             int32_t pz, const StencilCodes& c) {
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(occ), (short)0, 0x7FFFFFFF, 0x00020000);
-    // 4^3 blocks of the clamped stencil [p - 1, p + 1] per axis: b0 and b1
-    const int32_t bx0 = (px > 0 ? px - 1 : 0) >> 2, bx1 = (px + 1 < N ? px + 1 : N - 1) >> 2;
-    const int32_t by0 = (py > 0 ? py - 1 : 0) >> 2, by1 = (py + 1 < N ? py + 1 : N - 1) >> 2;
-    const int32_t bz0 = (pz > 0 ? pz - 1 : 0) >> 2, bz1 = (pz + 1 < N ? pz + 1 : N - 1) >> 2;
+    const bool vx[3] = {px > 0, true, px + 1 < N}, vy[3] = {py > 0, true, py + 1 < N}, vz[3] = {pz > 0, true, pz + 1 < N};
+    // first and last stencil coordinate inside the grid, per axis
+    const uint32_t xs = vx[0] ? c.cx[0] : c.cx[1], xe = vx[2] ? c.cx[2] : c.cx[1];
+    const uint32_t ys = vy[0] ? c.cy[0] : c.cy[1], ye = vy[2] ? c.cy[2] : c.cy[1];
+    const uint32_t zs = vz[0] ? c.cz[0] : c.cz[1], ze = vz[2] ? c.cz[2] : c.cz[1];
     uint64_t W[2][2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const uint32_t wi = morton3(a ? bx1 : bx0, b ? by1 : by0, c ? bz1 : bz0);
+            for (int d = 0; d < 2; ++d) {
+                const uint32_t wi = ((a ? xe : xs) | (b ? ye : ys) | (d ? ze : zs)) >> 6;
                 const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * 8u), 0, 0));
-                W[a][b][c] = ((uint64_t)w.y << 32) | w.x;
+                W[a][b][d] = ((uint64_t)w.y << 32) | w.x;
             }
-    // inside a block, voxel (x, y, z) is bit morton3(x & 3, y & 3, z & 3)
-    const auto low = [](int32_t v, int sh) { return (uint32_t)(((v & 1) << sh) | ((v & 2) << (sh + 2))); };
     uint32_t mask = 0;
 #pragma unroll
     for (int xo = 0; xo < 3; ++xo) {
-        const int32_t x = px - 1 + xo;
-        if ((uint32_t)x >= (uint32_t)N) continue;
-        const int a = (x >> 2) != bx0;
+        if (!vx[xo]) continue;
+        const int a = (c.cx[xo] >> 6) != (xs >> 6);
 #pragma unroll
         for (int yo = 0; yo < 3; ++yo) {
-            const int32_t y = py - 1 + yo;
-            if ((uint32_t)y >= (uint32_t)N) continue;
-            const int b = (y >> 2) != by0;
+            if (!vy[yo]) continue;
+            const int b = (c.cy[yo] >> 6) != (ys >> 6);
             const uint64_t w0 = a ? (b ? W[1][1][0] : W[1][0][0]) : (b ? W[0][1][0] : W[0][0][0]);
             const uint64_t w1 = a ? (b ? W[1][1][1] : W[1][0][1]) : (b ? W[0][1][1] : W[0][0][1]);
-            const uint32_t mxy = low(x, 0) | low(y, 1);
+            const uint32_t cxy = c.cx[xo] | c.cy[yo];
 #pragma unroll
             for (int zo = 0; zo < 3; ++zo) {
-                const int32_t z = pz - 1 + zo;
-                if ((uint32_t)z >= (uint32_t)N) continue;
-                const uint64_t w = ((z >> 2) != bz0) ? w1 : w0;
-                mask |= (uint32_t)((w >> (mxy | low(z, 2))) & 1u) << (9 * xo + 3 * yo + zo);
+                if (!vz[zo]) continue;
+                const uint64_t w = ((c.cz[zo] >> 6) != (zs >> 6)) ? w1 : w0;
+                mask |= (uint32_t)((w >> ((cxy | c.cz[zo]) & 63u)) & 1u) << (9 * xo + 3 * yo + zo);
             }
         }
     }
@@ -731,12 +761,14 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
                                                const uint32_t* __restrict__ occ, int32_t N, float dx, float p1x,
                                                float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
     const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
-    float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
-    uint32_t mask = occ_mask(occ, N, px, py, pz);
+    const StencilCodes sc = stencil_codes(px, py, pz);
+    uint32_t mask = occ_mask(occ, N, px, py, pz, sc);
     if (mask == 0) return;
+    const auto pick = [](const uint32_t (&v)[3], uint32_t o) { return o == 0 ? v[0] : (o == 1 ? v[1] : v[2]); };
     const auto range_of = [&](uint32_t q) {
-        return se[morton3(px - 1 + (int)(q / 9), py - 1 + (int)((q / 3) % 3), pz - 1 + (int)(q % 3))];
+        return se[pick(sc.cx, q / 9) | pick(sc.cy, (q / 3) % 3) | pick(sc.cz, q % 3)];
     };
     int2 cur = range_of((uint32_t)__builtin_ctz(mask));
     mask &= mask - 1;
@@ -750,7 +782,7 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
         for (; j <= e; ++j) {
             // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
             const float4 ahead = load_nb(rs, j < e ? j + 1 : (more ? nxt.x : j));
-            force_pair(o, p1x, p1y, p1z, r1, thres, fx, fy, fz);
+            force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
             o = ahead;
         }
         if (!more) break;
@@ -948,11 +980,11 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_blocks(const gell_cell* 
                 // FMkernel.cuh:56-118 from the LDS copy
                 const int32_t lx = trunc_index(me.x / dx) - bx, ly = trunc_index(me.y / dx) - by,
                               lz = trunc_index(me.z / dx) - bz;
-                float thres = 25.f;  // Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+                Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
                 for (int q = 0; q < 27; ++q) {
                     const int h = ((lz + q % 3) * kHalo + (ly + (q / 3) % 3)) * kHalo + (lx + q / 9);
                     const int k0 = ho[h], k1 = k0 + hc[h];
-                    for (int k = k0; k < k1; ++k) force_pair(cand[k], me.x, me.y, me.z, me.w, thres, fx, fy, fz);
+                    for (int k = k0; k < k1; ++k) force_pair(cand[k], me.x, me.y, me.z, me.w, cut, fx, fy, fz);
                 }
             } else {
                 cell_force<false>(nb, se, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
