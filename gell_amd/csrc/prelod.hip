@@ -6,7 +6,10 @@
 // atomics land in. Here every rate is turned into an exact 2^-32 fixed-point
 // integer and summed with 64-bit integer atomics (order-independent), then
 // rounded to float once. Only voxels that hold cells are touched: the voxels
-// the previous call wrote are cleared first (O(n) instead of O(N^3)).
+// the previous call wrote are cleared first (O(n) instead of O(N^3)) -- along
+// each x line's span of nonzero rates, with coalesced stores, when the spans
+// of the previous map are at hand, else voxel by voxel from the previous
+// call's list. (A voxel's accumulator is nonzero only where its rate is.)
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -23,6 +26,22 @@ __global__ __launch_bounds__(kBlock) void prelod_clear(const int32_t* __restrict
     if (v >= 0) {
         acc[v] = 0;
         rate[v] = 0.f;
+    }
+}
+
+// The same from the x-line spans of the map (every nonzero rate lies inside
+// its line's span): one wave per line, consecutive lanes on consecutive voxels.
+__global__ __launch_bounds__(kBlock) void prelod_clear_spans(const int2* __restrict__ xspan,
+                                                             long long* __restrict__ acc, float* __restrict__ rate,
+                                                             int32_t N) {
+    const int64_t line = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    if (line >= (int64_t)N * N) return;  // uniform per wave
+    const int2 sp = xspan[line];
+    float* r = rate + line * N;
+    long long* a = acc + line * N;
+    for (int i = sp.x + lane_id(); i <= sp.y; i += 64) {
+        a[i] = 0;
+        r[i] = 0.f;
     }
 }
 
@@ -95,7 +114,11 @@ __global__ __launch_bounds__(kBlock) void xspan_scan(const float* __restrict__ r
 hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n) {
     hipStream_t st = ctx->stream;
     const int32_t m = ctx->rate_list_n[s];
-    if (m > 0)
+    const int64_t lines = (int64_t)ctx->N * ctx->N;
+    if (m > 0 && ctx->xspan_valid[s])
+        hipLaunchKernelGGL(prelod_clear_spans, dim3((unsigned)((lines + kBlock / 64 - 1) / (kBlock / 64))),
+                           dim3(kBlock), 0, st, ctx->xspan[s], ctx->acc[s], ctx->rate[s], ctx->N);
+    else if (m > 0)
         hipLaunchKernelGGL(prelod_clear, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ctx->rate_list[s], m,
                            ctx->acc[s], ctx->rate[s]);
     ctx->rate_list_n[s] = 0;

@@ -34,6 +34,42 @@ def test_prelod_matches_oracle(n_sub, oracle):
     st.close()
 
 
+def test_prelod_span_clear_matches_oracle(oracle):
+    """From the second mechanics step on, PreLOD clears the previous map
+    along its x-line spans (rebuilt by the substeps in between) instead of
+    voxel by voxel. After four steps of a moving spheroid the maps equal the
+    oracle's bit for bit (a voxel left uncleared would show), and cells and
+    fields equal those of a run that clears voxel by voxel (handing out the
+    map pointer before each PreLOD drops the spans)."""
+    from gell_amd import spheroid
+
+    N, n = 128, 30000
+    P = params(N, capacity=n, n_sub=2)
+    cells = spheroid.make_cells(n, N, 10.0, 400.0, 3, capacity=n)
+    outs = []
+    for listed in (False, True):
+        st = GpuState(P, cells)
+        for _ in range(4):
+            st.ctx.fm_update(st.buf, n)
+            for s in range(2):
+                if listed:
+                    st.ctx.lib.gell_uptake_field(st.ctx.h, s)
+                st.ctx.prelod(s, st.buf, n)
+            for s in range(2):
+                for _ in range(3):
+                    st.ctx.lod_substep(s)
+        got_cells = st.cells()[:n].copy()
+        maps = [st.ctx.uptake_field(s) for s in range(2)]
+        fields = [st.ctx.field(s) for s in range(2)]
+        st.close()
+        for s in range(2):
+            want = oracle.prelod(P, s, got_cells, n)
+            assert np.array_equal(maps[s].view(np.uint32), want.view(np.uint32)), f"listed={listed} sub {s}"
+        outs.append([got_cells.tobytes()] + [f.tobytes() for f in fields])
+    assert outs[0] == outs[1]
+    log("[prelod] span clear ok")
+
+
 def _birth_state(N, n, capacity, seed):
     cells = mixed_cells(N, 10.0, n, capacity, seed=seed, outside=0)
     rng = np.random.default_rng(seed)
