@@ -550,9 +550,10 @@ __device__ __forceinline__ void x_lines_solve(float (&v)[N / P], const float* __
     const int sbase = (int)(L0 * N * 4) + k * m * 4;
     // With at most two chunks every load of the segment is issued before the
     // first transpose, so the second chunk does not wait for a round trip of
-    // its own behind the first chunk's barriers (the registers are free: LDS
-    // already limits these kernels to two waves per SIMD).
-    constexpr bool kPre = NC <= 2;
+    // its own behind the first chunk's barriers. Not with eight waves per
+    // workgroup (N = 512): the preload's 128 registers would leave room for
+    // one such workgroup per CU instead of two.
+    constexpr bool kPre = NC <= 2 && P < 8;
     int2 sp[Q];
 #pragma unroll
     for (int it = 0; it < Q; ++it) sp[it] = xspan[L0 + it * LPI + lq];
