@@ -97,7 +97,7 @@ def read_traffic(name: str):
     return None
 
 
-def gpu_leg(cfg, P, cells, steps, warmup, graph=False):
+def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
     """Timed biology steps of the workload. With graph, the steps run through
     the library's graph path (a one-replicate ensemble: each mechanics step
     after the first of a biology step is one hipGraph launch); otherwise
@@ -114,6 +114,9 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False):
 
     tuned = ctx.autotune()
     log(f"[bench] autotuned: {tuned}")
+    for k, v in (knobs or {}).items():  # --knob overrides the autotuner's choice
+        ctx.tune(k, v)
+        tuned[k] = v
 
     # The substrates do not interact inside a mechanics step (each substep
     # reads only its own field and uptake map), so running all substeps of
@@ -527,7 +530,7 @@ def main() -> int:
     ap.add_argument("--only", choices=["lod", "fm", "copy", "surrogate"], default=None,
                     help="run one kernel group alone (profiler runs); prints its timings")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
-                    help="kernel-variant knob for --only runs (e.g. lod_parts=8)")
+                    help="kernel-variant knob (e.g. lod_parts=8); in a full run it overrides the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the library has no multi-GPU path",
@@ -543,11 +546,11 @@ def main() -> int:
     cfg = CONFIGS[args.config]
     log(f"[bench] building workload {args.config}")
     P, cells = make_workload(cfg, args.seed)
+    knobs = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.knob)}
     if args.only:
-        knobs = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.knob)}
         print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup, knobs)), flush=True)
         return 0
-    g = gpu_leg(cfg, P, cells, args.steps, args.warmup, graph=cfg.get("graph", False) or args.graph)
+    g = gpu_leg(cfg, P, cells, args.steps, args.warmup, graph=cfg.get("graph", False) or args.graph, knobs=knobs)
 
     secs = g["seconds"]
     V = cfg["N"] ** 3
