@@ -88,7 +88,6 @@ void release(gell_ctx* c) {
         }
         if (c->sub_fork[s]) (void)hipEventDestroy(c->sub_fork[s]);
         if (c->sub_join[s]) (void)hipEventDestroy(c->sub_join[s]);
-        if (c->sub_read[s]) (void)hipEventDestroy(c->sub_read[s]);
     }
     for (auto& m : c->marks) {
         (void)hipEventDestroy(m.a);
@@ -250,7 +249,6 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         A(hipStreamCreateWithFlags(&c->sub_stream[s], hipStreamNonBlocking));
         A(hipEventCreateWithFlags(&c->sub_fork[s], hipEventDisableTiming));
         A(hipEventCreateWithFlags(&c->sub_join[s], hipEventDisableTiming));
-        A(hipEventCreateWithFlags(&c->sub_read[s], hipEventDisableTiming));
     }
     if (e != hipSuccess) {
         release(c);
@@ -318,26 +316,17 @@ int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int3
     if (substrate < 0 || substrate >= ctx->P.n_substrates) return fail(ctx, GELL_E_ARG, "substrate out of range");
     int r = check_cells(ctx, d_cells, n);
     if (r) return r;
-    hipError_t e = hipSuccess;
-    if (gell::substrate_stream_on(ctx, substrate)) {
-        // On the substrate's own stream, behind its previous substeps (which
-        // read the map about to be rewritten) and after the context stream's
-        // work so far (the cells' last writer); the context stream then waits
-        // for the map's reads of the cells before anything later writes them.
-        const hipStream_t st = ctx->sub_stream[substrate];
-        e = hipEventRecord(ctx->sub_fork[substrate], ctx->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, ctx->sub_fork[substrate], 0);
-        if (e == hipSuccess) e = gell::prelod_launch(ctx, substrate, d_cells, n, st);
-        if (e == hipSuccess) e = hipEventRecord(ctx->sub_read[substrate], st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->sub_read[substrate], 0);
-        ctx->sub_forked[substrate] = false;
-        ctx->sub_pending[substrate] = e == hipSuccess;  // the substeps follow on st
-    } else {
-        // the substrate's previous substeps read the uptake map about to be rewritten
-        e = gell::join_substrate(ctx, substrate);
-        if (e != hipSuccess) return hip_fail(ctx, e, "prelod");
+    // the substrate's previous substeps read the uptake map about to be rewritten
+    hipError_t e = gell::join_substrate(ctx, substrate);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prelod");
+    {
         PhaseScope ps(ctx, gell::PH_PRELOD);
-        e = gell::prelod_launch(ctx, substrate, d_cells, n, ctx->stream);
+        e = gell::prelod_launch(ctx, substrate, d_cells, n);
+    }
+    // fork point of the substrate's own stream: right after its uptake map
+    if (e == hipSuccess && gell::substrate_stream_on(ctx, substrate)) {
+        e = hipEventRecord(ctx->sub_fork[substrate], ctx->stream);
+        ctx->sub_forked[substrate] = e == hipSuccess;
     }
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "prelod");
 }

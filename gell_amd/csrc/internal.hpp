@@ -71,18 +71,14 @@ struct gell_ctx {
     // Kernel-variant knobs (GELL_KNOB_*).
     int tune[GELL_N_KNOBS] = {1, 6, 4, 4, 1, 1};
 
-    // Every substrate runs its uptake map and LOD substeps on a stream of its
-    // own when GELL_KNOB_SUBSTRATE_STREAMS is set: the uptake map waits for
-    // the context stream's work so far (the cells' last writer) and the
-    // context stream waits for the map's reads of the cells; the substeps
-    // follow on that stream (or fork at the first substep when no map was
-    // made there), joined back before anything else touches that substrate
-    // (see gell::join_substrate). The mechanics of the next step, which read
-    // cells only, overlap them, and the substrates' maps overlap each other.
+    // Every substrate runs its LOD substeps on a stream of its own when
+    // GELL_KNOB_SUBSTRATE_STREAMS is set: forked from the context stream after
+    // its uptake map (or at its first substep), joined back before anything
+    // else touches that substrate (see gell::join_substrate). The mechanics
+    // of the next step, which read cells only, overlap them.
     hipStream_t sub_stream[GELL_MAX_SUBSTRATES] = {};
     hipEvent_t sub_fork[GELL_MAX_SUBSTRATES] = {};
     hipEvent_t sub_join[GELL_MAX_SUBSTRATES] = {};
-    hipEvent_t sub_read[GELL_MAX_SUBSTRATES] = {};  // PreLOD on the substrate stream has read the cells
     bool sub_forked[GELL_MAX_SUBSTRATES] = {};   // fork event recorded, not yet waited on
     bool sub_pending[GELL_MAX_SUBSTRATES] = {};  // work on sub_stream not yet joined
     // y-coupling values of the fused LOD substep, one buffer per substrate
@@ -126,7 +122,7 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n);
 hipError_t fm_init_ranges(gell_ctx* ctx);
 hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out);
 
-hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n, hipStream_t st);
+hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n);
 
 hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out);
 hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out);

@@ -111,7 +111,8 @@ __global__ __launch_bounds__(kBlock) void xspan_scan(const float* __restrict__ r
 
 }  // namespace
 
-hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n, hipStream_t st) {
+hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n) {
+    hipStream_t st = ctx->stream;
     const int32_t m = ctx->rate_list_n[s];
     const int64_t lines = (int64_t)ctx->N * ctx->N;
     if (m > 0 && ctx->xspan_valid[s])

@@ -216,11 +216,10 @@ int gell_sync(gell_ctx* ctx);
  *                              with serial lines, 1 = the x and y sweeps of
  *                              each z plane in one workgroup, bit-identical
  *                              to the separate sweeps. Default 1.
- *   GELL_KNOB_SUBSTRATE_STREAMS 1 = each substrate's uptake map and LOD
- *                              substeps run on a stream of its own,
- *                              overlapping the other substrates' and the
- *                              next mechanics step (same results; off while
- *                              profiling).
+ *   GELL_KNOB_SUBSTRATE_STREAMS 1 = each substrate's LOD substeps run on a
+ *                              stream of its own, overlapping the other
+ *                              substrates' substeps and the next mechanics
+ *                              step (same results; off while profiling).
  *                              Field and uptake-map pointers are current
  *                              after gell_sync. Default 1.
  * The first two knobs never change results; gell_autotune sets all of them. */
