@@ -8,10 +8,11 @@
  * with gell_last_error().
  *
  * Threading: one context is driven by one host thread; all work is enqueued
- * on the context's HIP stream. Entry points that return a host-side count
- * (gell_birth, gell_death) synchronise that stream; the others are
- * asynchronous. Cell arrays are caller-owned device memory; all scratch is
- * context-owned.
+ * on the context's HIP stream, except that each substrate's LOD substeps may
+ * run on a context-owned stream ordered against it
+ * (GELL_KNOB_SUBSTRATE_STREAMS below). Entry points that return a host-side
+ * count (gell_birth, gell_death) synchronise; the others are asynchronous.
+ * Cell arrays are caller-owned device memory; all scratch is context-owned.
  *
  * Reference interfaces replaced (paths relative to the Gell sources):
  *   gell_fm_update   <- MechanicsMesh_struct::FM_update    include/FMkernel.cuh:181-202
