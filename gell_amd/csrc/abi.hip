@@ -381,12 +381,27 @@ int gell_death(gell_ctx* ctx, gell_cell* d_cells, int32_t n, int32_t* n_out) {
     return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "death");
 }
 
+// Both accessors order the substrate's own stream back into the context
+// stream first (so work the caller enqueues on the context stream, or on the
+// null stream when the context was created on it, sees the finished field).
 float* gell_field(gell_ctx* ctx, int32_t s) {
-    return (ctx && s >= 0 && s < ctx->P.n_substrates) ? ctx->field[s] : nullptr;
+    if (!ctx || s < 0 || s >= ctx->P.n_substrates) return nullptr;
+    if (gell::join_substrate(ctx, s) != hipSuccess) {
+        hip_fail(ctx, hipGetLastError(), "gell_field");
+        return nullptr;
+    }
+    return ctx->field[s];
 }
 float* gell_uptake_field(gell_ctx* ctx, int32_t s) {
     if (!ctx || s < 0 || s >= ctx->P.n_substrates) return nullptr;
-    ctx->xspan_valid[s] = false;  // the caller may write the map: rebuild the line spans before use
+    if (gell::join_substrate(ctx, s) != hipSuccess) {
+        hip_fail(ctx, hipGetLastError(), "gell_uptake_field");
+        return nullptr;
+    }
+    // The caller may write the map: the line spans are rebuilt before use and
+    // the next PreLOD clears the whole map.
+    ctx->xspan_valid[s] = false;
+    ctx->rate_handed_out[s] = true;
     return ctx->rate[s];
 }
 const int32_t* gell_sorted_keys(gell_ctx* ctx) { return ctx ? ctx->out_keys : nullptr; }

@@ -58,6 +58,12 @@ struct gell_ctx {
     long long* acc[GELL_MAX_SUBSTRATES] = {};
     int32_t* rate_list[GELL_MAX_SUBSTRATES] = {};
     int32_t rate_list_n[GELL_MAX_SUBSTRATES] = {};
+    // Set when gell_uptake_field handed out the map pointer since the last
+    // PreLOD: the caller may have written any voxel, so neither the voxel list
+    // nor the x-line spans bound the nonzero rates and accumulators any more,
+    // and the next PreLOD clears the whole map (as the reference's
+    // oxy_consumption_reset does every time, LODkernel.cuh:272).
+    bool rate_handed_out[GELL_MAX_SUBSTRATES] = {};
 
     // Proliferation / death.
     int32_t* flags = nullptr;      // per slot

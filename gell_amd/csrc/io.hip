@@ -202,6 +202,19 @@ int gell_checkpoint_load(gell_ctx* ctx, gell_cell* d_cells, int32_t* n, uint64_t
     const gell_params& P = ctx->P;
     if (h.params.N != P.N || h.params.n_substrates != P.n_substrates || h.params.dx != P.dx)
         return io_fail(GELL_E_ARG, "checkpoint grid or substrate count differ from this context's");
+    // Everything else a step's results depend on must match too, or the
+    // restarted run would silently leave the uninterrupted one's trajectory.
+    // (capacity and each substrate's initial value do not enter a step.)
+    if (h.params.seed != P.seed)
+        return io_fail(GELL_E_ARG, "checkpoint seed differs from this context's");
+    if (h.params.dt_bio != P.dt_bio || h.params.dt_mech != P.dt_mech || h.params.dt_diff != P.dt_diff)
+        return io_fail(GELL_E_ARG, "checkpoint time steps differ from this context's");
+    for (int s = 0; s < P.n_substrates; ++s) {
+        const gell_substrate &a = h.params.sub[s], &b = P.sub[s];
+        if (a.D != b.D || a.decay != b.decay || a.uptake != b.uptake || a.dead_factor != b.dead_factor ||
+            a.boundary != b.boundary)
+            return io_fail(GELL_E_ARG, "checkpoint substrate constants differ from this context's");
+    }
     if (h.n < 0 || h.n > P.capacity) return io_fail(GELL_E_ARG, "checkpoint holds more cells than this context's capacity");
     // the whole file is read before anything on the device changes
     std::vector<gell_cell> cells((size_t)h.n);

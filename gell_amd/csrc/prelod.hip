@@ -10,6 +10,8 @@
 // each x line's span of nonzero rates, with coalesced stores, when the spans
 // of the previous map are at hand, else voxel by voxel from the previous
 // call's list. (A voxel's accumulator is nonzero only where its rate is.)
+// After the map pointer was handed out to the caller, who may have written
+// any voxel, the whole map and its accumulators are cleared instead.
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -42,6 +44,17 @@ __global__ __launch_bounds__(kBlock) void prelod_clear_spans(const int2* __restr
     for (int i = sp.x + lane_id(); i <= sp.y; i += 64) {
         a[i] = 0;
         r[i] = 0.f;
+    }
+}
+
+// Whole-map clear (no hipMemsetAsync: the ensemble captures this launch into
+// a graph).
+__global__ __launch_bounds__(kBlock) void prelod_clear_all(long long* __restrict__ acc, float* __restrict__ rate,
+                                                           int64_t V) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) {
+        acc[i] = 0;
+        rate[i] = 0.f;
     }
 }
 
@@ -115,13 +128,17 @@ hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n
     hipStream_t st = ctx->stream;
     const int32_t m = ctx->rate_list_n[s];
     const int64_t lines = (int64_t)ctx->N * ctx->N;
-    if (m > 0 && ctx->xspan_valid[s])
+    if (ctx->rate_handed_out[s])
+        hipLaunchKernelGGL(prelod_clear_all, dim3((unsigned)((ctx->V + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                           ctx->acc[s], ctx->rate[s], ctx->V);
+    else if (m > 0 && ctx->xspan_valid[s])
         hipLaunchKernelGGL(prelod_clear_spans, dim3((unsigned)((lines + kBlock / 64 - 1) / (kBlock / 64))),
                            dim3(kBlock), 0, st, ctx->xspan[s], ctx->acc[s], ctx->rate[s], ctx->N);
     else if (m > 0)
         hipLaunchKernelGGL(prelod_clear, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ctx->rate_list[s], m,
                            ctx->acc[s], ctx->rate[s]);
     ctx->rate_list_n[s] = 0;
+    ctx->rate_handed_out[s] = false;
     if (n > 0) {
         const gell_substrate& sb = ctx->P.sub[s];
         const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);

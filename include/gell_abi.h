@@ -138,7 +138,13 @@ int gell_birth(gell_ctx* ctx, gell_cell* d_cells, int32_t n, uint64_t bio_step, 
 /* Stable removal of records with sign == 0 from d_cells[0, n). Synchronises. */
 int gell_death(gell_ctx* ctx, gell_cell* d_cells, int32_t n, int32_t* n_out);
 
-/* Device pointers owned by the context (valid until gell_destroy). */
+/* Device pointers owned by the context (valid until gell_destroy). Both
+ * first order the substrate's own stream (GELL_KNOB_SUBSTRATE_STREAMS) back
+ * into the context stream, so work enqueued afterwards on the context stream
+ * (or on the null stream, for a context made on it) sees the finished field;
+ * a caller on any other stream calls gell_sync first. The caller may write
+ * the uptake map: the next gell_prelod of that substrate then clears the
+ * whole map instead of only the voxels its previous call wrote. */
 float* gell_field(gell_ctx* ctx, int32_t substrate);        /* N^3 floats, x fastest */
 float* gell_uptake_field(gell_ctx* ctx, int32_t substrate); /* N^3 floats (p_rate) */
 
@@ -314,8 +320,10 @@ int gell_write_gap_counts_csv(const int32_t* counts, const double* gap_times, in
  * uptake maps are rebuilt by every mechanics step; the birth draws are keyed
  * on the step counter), so a run restarted from a checkpoint continues bit
  * for bit as the uninterrupted run. Load needs a context with the same N,
- * dx and substrate count and room for the saved records; it writes the
- * records to d_cells and the count and step to *n and *bio_step.
+ * dx, substrate count, seed, time steps and substrate constants (D, decay,
+ * uptake, dead factor, boundary) and room for the saved records, and refuses
+ * the file otherwise; it writes the records to d_cells and the count and step
+ * to *n and *bio_step.
  * Both synchronise; a failed load changes nothing on the device. */
 int gell_checkpoint_save(gell_ctx* ctx, const gell_cell* d_cells, int32_t n, uint64_t bio_step, const char* path);
 int gell_checkpoint_load(gell_ctx* ctx, gell_cell* d_cells, int32_t* n, uint64_t* bio_step, const char* path);

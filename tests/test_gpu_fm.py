@@ -117,3 +117,31 @@ def test_fm_bad_count_is_an_error():
     with pytest.raises(rt.GellError):
         st.ctx.fm_update(st.buf, 9)
     st.close()
+
+
+def test_fm_carried_threshold_unequal_radii(oracle):
+    """The hand-built clusters whose accepted pairs the carried threshold
+    decides (FMkernel.cuh:72, 91, 98-99): every force kernel variant against
+    the oracle bit for bit and against the hand-derived float64 forces."""
+    from test_oracle_fm import CARRY_ACCEPTED, carry_case, carry_expected
+
+    N, dx, cells = carry_case()
+    n = len(cells)
+    P = params(N, dx=dx, capacity=n)
+    want = cells.copy()
+    _, order, _, _ = oracle.fm_update(P, want, n)
+    sorted_in = cells[order].copy()
+    oracle.fm_keys(P, sorted_in, n)
+    hand = carry_expected(sorted_in, CARRY_ACCEPTED)
+    st = GpuState(P, cells)
+    for v in range(7):
+        log(f"[fm] carried thres, force variant {v}")
+        st.ctx.tune("fm_prefetch", v)
+        st.upload(cells)
+        st.ctx.fm_update(st.buf, n)
+        got = st.cells()
+        compare_cells(got, want, n)
+        f = got["oldforce"][:n].astype(np.float64)
+        for i in range(n):
+            assert np.linalg.norm(f[i] - hand[i]) <= 2e-5 * np.linalg.norm(hand[i]), (v, i)
+    st.close()

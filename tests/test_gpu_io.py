@@ -91,6 +91,17 @@ def test_checkpoint_refuses_other_grid_and_other_files(tmp_path):
     other = GpuState(params(40, capacity=1000), cells)
     with pytest.raises(rt.GellError, match="grid"):
         other.ctx.checkpoint_load(other.buf, tmp_path / "a.bin")
+    other_seed = GpuState(params(32, capacity=1000, seed=124), cells)
+    with pytest.raises(rt.GellError, match="seed"):
+        other_seed.ctx.checkpoint_load(other_seed.buf, tmp_path / "a.bin")
+    other_dt = GpuState(params(32, capacity=1000, dt_mech=0.05), cells)
+    with pytest.raises(rt.GellError, match="time steps"):
+        other_dt.ctx.checkpoint_load(other_dt.buf, tmp_path / "a.bin")
+    p_sub = params(32, capacity=1000)
+    p_sub.sub[0].uptake = -5.0
+    other_sub = GpuState(p_sub, cells)
+    with pytest.raises(rt.GellError, match="substrate constants"):
+        other_sub.ctx.checkpoint_load(other_sub.buf, tmp_path / "a.bin")
     (tmp_path / "junk.bin").write_bytes(b"not a checkpoint" * 100)
     with pytest.raises(rt.GellError, match="format"):
         a.ctx.checkpoint_load(a.buf, tmp_path / "junk.bin")
@@ -98,7 +109,7 @@ def test_checkpoint_refuses_other_grid_and_other_files(tmp_path):
     (tmp_path / "cut.bin").write_bytes(raw[: len(raw) // 2])
     with pytest.raises(rt.GellError, match="truncated"):
         a.ctx.checkpoint_load(a.buf, tmp_path / "cut.bin")
-    for st in (a, small, other):
+    for st in (a, small, other, other_seed, other_dt, other_sub):
         st.close()
 
 

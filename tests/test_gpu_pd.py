@@ -39,8 +39,8 @@ def test_prelod_span_clear_matches_oracle(oracle):
     along its x-line spans (rebuilt by the substeps in between) instead of
     voxel by voxel. After four steps of a moving spheroid the maps equal the
     oracle's bit for bit (a voxel left uncleared would show), and cells and
-    fields equal those of a run that clears voxel by voxel (handing out the
-    map pointer before each PreLOD drops the spans)."""
+    fields equal those of a run that clears the whole map (handing out the
+    map pointer before each PreLOD makes the next PreLOD clear all of it)."""
     from gell_amd import spheroid
 
     N, n = 128, 30000
@@ -82,6 +82,46 @@ def _birth_state(N, n, capacity, seed):
     live["Vcs"] = np.where(rng.random(n) < 0.1, 20.0, live["Vcs"]).astype(np.float32)
     p0 = (rng.random(N ** 3) * 40.0).astype(np.float32)
     return cells, p0
+
+
+@pytest.mark.parametrize("edit", ["zero_region", "write_outside"])
+def test_prelod_after_caller_wrote_the_map(edit, oracle):
+    """The caller may write the uptake map through the pointer
+    gell_uptake_field hands out (the reference's public p_rate). Either it
+    zeroes a slab that holds tissue between a PreLOD and the substeps (which
+    then rebuild the x-line spans from the edited map; the spans no longer
+    cover every nonzero accumulator), or it writes rates into voxels that
+    hold no cell after the substeps (outside the previous voxel list). The
+    next PreLOD must still give the oracle's map bit for bit: it clears the
+    whole map, as the reference's oxy_consumption_reset (LODkernel.cuh:272)
+    always does."""
+    from gell_amd import spheroid
+
+    N, n = 128, 30000
+    P = params(N, capacity=n, n_sub=1)
+    cells = spheroid.make_cells(n, N, 10.0, 400.0, 5, capacity=n)
+    st = GpuState(P, cells)
+
+    def write_map(fn):
+        m = st.ctx.uptake_field(0).reshape(N, N, N)
+        fn(m)
+        rt.upload_ptr(st.ctx.lib.gell_uptake_field(st.ctx.h, 0), m.ravel())
+
+    for step in range(2):
+        st.ctx.fm_update(st.buf, n)
+        st.ctx.prelod(0, st.buf, n)
+        if step == 1 and edit == "zero_region":
+            write_map(lambda m: m.__setitem__(slice(N // 2 - 8, N // 2 + 8), 0.0))
+        for _ in range(3):
+            st.ctx.lod_substep(0)
+        if step == 1 and edit == "write_outside":
+            write_map(lambda m: m.__setitem__(slice(0, 4), -3.0))  # z < 4: far outside the ball
+    st.ctx.fm_update(st.buf, n)
+    st.ctx.prelod(0, st.buf, n)
+    got_cells = st.cells()
+    want = oracle.prelod(P, 0, got_cells, n)
+    assert np.array_equal(st.ctx.uptake_field(0).view(np.uint32), want.view(np.uint32)), edit
+    st.close()
 
 
 @pytest.mark.parametrize("capacity_room", [None, 37])

@@ -160,3 +160,115 @@ def test_move_closed_form_and_domain_exit(oracle):
         assert np.array_equal(got["pos"][i], cells["pos"][i])
         assert np.array_equal(got["force"][i], cells["force"][i])
         assert np.array_equal(got["oldforce"][i], cells["oldforce"][i])
+
+
+# ---------------------------------------------------------------------------
+# Carried threshold with unequal radii (FMkernel.cuh:72, 91, 98-99)
+# ---------------------------------------------------------------------------
+# forcekernel keeps `thres` from the last pair that passed the first distance
+# test: it starts at Tumor_CC_Max_Dist (25), each such pair sets it to
+# 1.25 (r_i + r_j), and the first test of the next candidate (:91) compares
+# against that carried value before the candidate's own radii are looked at.
+# The stencil is walked x outer, z inner, slots ascending, and the self pair
+# (distance 0.01) is visited like any other. The two clusters below are laid
+# out so that the carried value decides one pair in each; the accepted sets
+# are worked out by hand in the comments, and the forces are then summed in
+# float64 from the pair formula (FMkernel.cuh:101-109).
+#
+# Cluster 1 (around voxel (3, 3, 3)):
+#   A r=5.0 (35, 35, 35)     voxel (3,3,3)
+#   B r=5.0 (35, 35, 46.5)   voxel (3,3,4): 11.5 from A
+#   C r=5.5 (47.8, 35, 35)   voxel (4,3,3): 12.8 from A, 17.2 from B
+#   A's walk: self (stencil index 13) -> thres 12.5; B (index 14) at 11.5 is
+#     accepted -> thres 12.5; C (index 22) at 12.8 >= 12.5 is rejected at :91.
+#     With its own radii C would have thres 13.125 > 12.8: the carried value
+#     decides.  A <- {B}.
+#   B's walk: A (index 12) at 11.5 < 25 -> thres 12.5, accepted; self; C far.
+#     B <- {A}.
+#   C's walk: A (index 4) at 12.8 < 25 -> thres 13.125 > 12.8, accepted; self.
+#     C <- {A}: the pair is accepted one way and not the other.
+# Cluster 2 (around voxel (11, 11, 11)):
+#   A2 r=5.5 (115, 115, 115) voxel (11,11,11)
+#   F  r=4.5 (115, 127, 115) voxel (11,12,11): 12.0 from A2
+#   G  r=5.5 (128, 115, 115) voxel (12,11,11): 13.0 from A2, 17.7 from F
+#   A2's walk: self -> thres 13.75; F (index 16) at 12.0 < 13.75 passes :91,
+#     its own thres is 12.5 > 12.0: accepted, thres 12.5; G (index 22) at
+#     13.0 >= 12.5 is rejected at :91 (its own thres would be 13.75).
+#     A2 <- {F}.
+#   F's walk: A2 (index 10) at 12.0 -> thres 12.5, accepted; self; G far.
+#   G's walk: A2 (index 4) at 13.0 < 25 -> thres 13.75, accepted; F (index 7)
+#     at 17.7 rejected; self.  G <- {A2}.
+CARRY_CELLS = [  # name, r, position
+    ("A", 5.0, (35.0, 35.0, 35.0)),
+    ("B", 5.0, (35.0, 35.0, 46.5)),
+    ("C", 5.5, (47.8, 35.0, 35.0)),
+    ("A2", 5.5, (115.0, 115.0, 115.0)),
+    ("F", 4.5, (115.0, 127.0, 115.0)),
+    ("G", 5.5, (128.0, 115.0, 115.0)),
+]
+CARRY_ACCEPTED = {"A": ["B"], "B": ["A"], "C": ["A"], "A2": ["F"], "F": ["A2"], "G": ["A2"]}
+# what an implementation that tests every candidate against its own radii
+# only would accept instead
+NO_CARRY_ACCEPTED = {"A": ["B", "C"], "B": ["A"], "C": ["A"], "A2": ["F", "G"], "F": ["A2"], "G": ["A2"]}
+
+
+def carry_case():
+    """Cells of the two clusters (N = 16, dx = 10), unsorted on purpose."""
+    N, dx = 16, 10.0
+    order = [4, 2, 0, 5, 3, 1]
+    cells = np.zeros(len(CARRY_CELLS), rt.CELL_DTYPE)
+    for slot, q in enumerate(order):
+        _, r, p = CARRY_CELLS[q]
+        cells["sign"][slot] = 1
+        cells["pos"][slot] = p
+        cells["r"][slot] = r
+        cells["V"][slot] = 2494.0
+    return N, dx, cells
+
+
+def pair_force64(p1, r1, p2, r2):
+    d3 = np.asarray(p1, np.float64) - np.asarray(p2, np.float64)
+    d = np.sqrt((d3 * d3).sum() + 1e-4)
+    R = max(r1 + r2, 0.01)
+    thres = 1.25 * R
+    assert d < thres
+    cr = 10.0 * (1 - d / R) ** 2 / d if d <= R else 0.0
+    ca = 0.4 * (1 - d / thres) ** 2 / d
+    return d3 * (cr - ca)
+
+
+def carry_expected(cells, accepted):
+    byname = {}
+    for slot in range(len(cells)):
+        p = tuple(float(x) for x in cells["pos"][slot])
+        name = next(nm for nm, _, q in CARRY_CELLS if np.allclose(q, p))
+        byname[name] = slot
+    f = np.zeros((len(cells), 3))
+    for name, others in accepted.items():
+        i = byname[name]
+        for o in others:
+            j = byname[o]
+            f[i] += pair_force64(cells["pos"][i], float(cells["r"][i]), cells["pos"][j], float(cells["r"][j]))
+    return f
+
+
+def test_force_carried_threshold_unequal_radii(oracle):
+    N, dx, cells = carry_case()
+    n = len(cells)
+    P = params(N, dx=dx, capacity=n)
+    work = cells.copy()
+    keys, order, s, e = oracle.fm_update(P, work, n)
+    sorted_cells = cells[order].copy()
+    oracle.fm_keys(P, sorted_cells, n)
+    f = oracle.forces(P, sorted_cells, n, s, e).astype(np.float64)
+    want = carry_expected(sorted_cells, CARRY_ACCEPTED)
+    alt = carry_expected(sorted_cells, NO_CARRY_ACCEPTED)
+    # near the cutoff (1 - d / thres) cancels: float32 rounding leaves a few
+    # 1e-6 relative; leaving out or adding one pair moves a force by > 9 %
+    for i in range(n):
+        assert np.linalg.norm(f[i] - want[i]) <= 2e-5 * np.linalg.norm(want[i]), (i, f[i], want[i])
+    moved = [i for i in range(n) if np.linalg.norm(alt[i] - want[i]) > 0.05 * np.linalg.norm(want[i])]
+    assert len(moved) == 2
+    # the step stores the force in oldforce (every cell stays inside)
+    assert np.all(work["sign"][:n] == 1)
+    assert np.array_equal(work["oldforce"][:n], f.astype(np.float32))
