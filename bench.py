@@ -35,11 +35,16 @@ sys.path.insert(0, str(ROOT))
 METRIC = "agent-steps/s & voxel-updates/s, 1M cells/256^3; surrogate vs LOD at 1e-3 L2"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per voxel-update of one LOD substep: three sweeps
-# x 12 (p, p_rate in; p out) + y 8 + z 8 = 28; the fused substep (x sweep and
-# y blocks in one pass, y coupling applied inside the z sweep) moves
-# 12 + 8 = 20.
+# x 12 (p, p_rate in; p out) + y 8 + z 8 = 28. The roofline is priced on these
+# 28 B whatever the variant; the fused substep (x sweep and y blocks in one
+# pass, y coupling applied inside the z sweep) moves only 12 + 8 = 20 of them,
+# which the counted traffic shows.
 LOD_BYTES_PER_VOXEL = 28
 LOD_BYTES_PER_VOXEL_FUSED = 20
+# Algorithmic bytes per agent-step of the force + movement kernel: the sorted
+# 72-byte record read once and written back once (neighbour reads are
+# expected to hit the caches and are not charged).
+FORCE_BYTES_PER_CELL = 144
 
 # Substrate constants: oxygen as in the reference's Definitions.h:21-25;
 # glucose with a smaller diffusivity, no decay and its own boundary value.
@@ -158,6 +163,7 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
     # same state, then one more biology step for birth and death.
     ctx.profile(True)
     ctx.profile_read(reset=True)
+    n_prof = n  # cells of each profiled mechanics step
     for _ in range(min(mech, 10)):
         ctx.fm_update(buf, n)
         for s in range(P.n_substrates):
@@ -169,7 +175,7 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
     prof = ctx.profile_read(reset=True)
     ctx.profile(False)
     out = dict(seconds=t1 - t0, agents=agents, n_final=n, mech=mech, diff=diff, prof=prof,
-               device=rt.device_name(), tuned=tuned)
+               device=rt.device_name(), tuned=tuned, n_prof=n_prof)
     buf.free()
     (ens if graph else ctx).close()
     return out
@@ -390,7 +396,100 @@ def ensemble_main(args) -> int:
     return 0
 
 
+def drift_main(args) -> int:
+    """Drift of the default (autotuned) GPU kernels over `--drift-steps`
+    mechanics steps of the workload (FM + uptake maps + 11 LOD substeps per
+    substrate, no PD) against the FP32 oracle (the parity reference) and the
+    FP64 oracle (the drift of FP32 arithmetic itself). All three runs start
+    from the same seeded state. The oracles run with OpenMP on this host's
+    cores, in chunks of 10 steps, printing their progress."""
+    from gell_amd import runtime as rt
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    cfg = CONFIGS[args.config]
+    P, cells = make_workload(cfg, args.seed)
+    n, N, K = cfg["cells"], cfg["N"], args.drift_steps
+    V = N ** 3
+    diff = rt.float_loop_count(P.dt_mech, P.dt_diff)
+    ctx = rt.Context(P)
+    tuned = ctx.autotune()
+    buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
+    buf.upload(cells)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ctx.fm_update(buf, n)
+        for s in range(P.n_substrates):
+            ctx.prelod(s, buf, n)
+        for s in range(P.n_substrates):
+            for _ in range(diff):
+                ctx.lod_substep(s)
+    ctx.sync()
+    t_gpu = time.perf_counter() - t0
+    g_cells = buf.download(rt.CELL_DTYPE, P.capacity)[:n]
+    g_fields = [ctx.field(s).astype(np.float64) for s in range(P.n_substrates)]
+    buf.free()
+    ctx.close()
+    log(f"[drift] GPU: {K} steps in {t_gpu:.2f} s")
+
+    orc = pyoracle.Oracle(omp=True)
+    runs = {}
+    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
+        work = cells.copy()
+        f = np.concatenate([np.full(V, P.sub[s].initial, dt) for s in range(P.n_substrates)])
+        t0 = time.perf_counter()
+        done = 0
+        while done < K:
+            k = min(10, K - done)
+            orc.run_mech(P, work, n, f, k)
+            done += k
+            log(f"[drift] oracle {name}: {done}/{K} steps, {time.perf_counter() - t0:.1f} s")
+        runs[name] = (work[:n], [f[s * V:(s + 1) * V].astype(np.float64) for s in range(P.n_substrates)],
+                      time.perf_counter() - t0)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    c32, f32, t32 = runs["fp32"]
+    c64, f64, t64 = runs["fp64"]
+    live = (c32["sign"] != 0) & (c64["sign"] != 0) & (g_cells["sign"] != 0)
+    pos = lambda c: c["pos"][live].astype(np.float64)  # noqa: E731
+    line = {
+        "metric": f"drift over {K} mechanics steps vs the FP32 and FP64 oracles (relative L2)",
+        "config": {"workload": f"{args.config}: {N}^3 voxels, {n} cells, {P.n_substrates} substrate(s); "
+                               f"step = FM + uptake maps + {diff} LOD substeps per substrate, no PD",
+                   "kernel_variants": tuned},
+        "steps": K,
+        "field_gpu_vs_fp32": [rel(g, w) for g, w in zip(g_fields, f32)],
+        "field_gpu_vs_fp64": [rel(g, w) for g, w in zip(g_fields, f64)],
+        "field_fp32_vs_fp64": [rel(a, b) for a, b in zip(f32, f64)],
+        "pos_gpu_vs_fp32": rel(pos(g_cells), pos(c32)),
+        "pos_gpu_vs_fp64": rel(pos(g_cells), pos(c64)),
+        "pos_fp32_vs_fp64": rel(pos(c32), pos(c64)),
+        "live_cells": {"gpu": int((g_cells["sign"] != 0).sum()), "fp32": int((c32["sign"] != 0).sum()),
+                       "fp64": int((c64["sign"] != 0).sum())},
+        "seconds": {"gpu": t_gpu, "oracle_fp32": t32, "oracle_fp64": t64},
+        "oracle_threads": orc.threads(),
+        "host_nproc": os.cpu_count(),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+# What the default run reports of the surrogate leg.
+SURROGATE_SUMMARY_KEYS = ("value", "unit", "predict_ms", "lod50_ms", "bio_step_ms", "speedup_bio_step",
+                          "field_rel_l2_vs_plain", "births", "deaths", "event_delta_frac", "surrogate_calls",
+                          "surrogate_accepted", "tolerance")
+
+
 def surrogate_main(args) -> int:
+    print(json.dumps(surrogate_leg(args)), flush=True)
+    return 0
+
+
+def surrogate_leg(args) -> dict:
     """Config N: the 256^3 / 1M-cell spheroid (oxygen), one biology step with
     plain LOD substeps against the same step with the surrogate standing in
     for each run of K = 50 substeps (uptake map of the mechanics step it
@@ -507,13 +606,16 @@ def surrogate_main(args) -> int:
         "births": {"plain": pl["births"], "surrogate": su["births"]},
         "deaths": {"plain": pl["deaths"], "surrogate": su["deaths"]},
         "lod_substeps": {"plain": pl["lod_substeps"], "surrogate": su["lod_substeps"]},
+        "speedup_bio_step": pl["seconds"] / su["seconds"],
+        "event_delta_frac": {
+            k: (abs(su[k] - pl[k]) / pl[k] if pl[k] else float(su[k] != pl[k])) for k in ("births", "deaths")},
+        "tolerance": args.sur_tol,
     }
-    print(json.dumps(line), flush=True)
     pred.free()
     buf.free()
     sur.close()
     ctx.close()
-    return 0
+    return line
 
 
 def main() -> int:
@@ -525,6 +627,11 @@ def main() -> int:
     ap.add_argument("--weights", default=str(SURROGATE_WEIGHTS), help="surrogate parameters (config N)")
     ap.add_argument("--graph", action="store_true", help="time through the graph path (one-replicate ensemble)")
     ap.add_argument("--sur-tol", type=float, default=1e-3, help="surrogate acceptance tolerance (config N)")
+    ap.add_argument("--no-surrogate", dest="surrogate", action="store_false",
+                    help="config L: skip the surrogate-vs-LOD leg")
+    ap.add_argument("--drift-steps", type=int, default=0,
+                    help="instead of timing: drift of the GPU run over this many mechanics steps against the FP32 "
+                         "and FP64 oracles (config L or S)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
     ap.add_argument("--only", choices=["lod", "fm", "copy", "surrogate"], default=None,
@@ -543,6 +650,8 @@ def main() -> int:
         return ensemble_main(args)
     if args.config == "N":
         return surrogate_main(args)
+    if args.drift_steps > 0:
+        return drift_main(args)
     cfg = CONFIGS[args.config]
     log(f"[bench] building workload {args.config}")
     P, cells = make_workload(cfg, args.seed)
@@ -559,10 +668,14 @@ def main() -> int:
     lod_us = lod_ms / max(lod_calls, 1) * 1e3
     tuned = g["tuned"]
     fused = bool(tuned.get("lod_fused")) and tuned["lod_parts"] > 1 and cfg["N"] in (128, 256, 512)
-    bpv = LOD_BYTES_PER_VOXEL_FUSED if fused else LOD_BYTES_PER_VOXEL
-    achieved = bpv * V / (lod_us * 1e-6) / 1e9
+    moved = LOD_BYTES_PER_VOXEL_FUSED if fused else LOD_BYTES_PER_VOXEL
+    achieved = LOD_BYTES_PER_VOXEL * V / (lod_us * 1e-6) / 1e9
     traffic = read_traffic(f"lod_substep_{cfg['N']}_traffic.txt")
     phases = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in g["prof"].items() if v[1]}
+    force_ms, force_calls = g["prof"]["fm_force_move"]
+    force_us = force_ms / max(force_calls, 1) * 1e3
+    force_achieved = FORCE_BYTES_PER_CELL * g["n_prof"] / (force_us * 1e-6) / 1e9
+    force_traffic = read_traffic(f"fm_{cfg['N']}_traffic.txt")
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -615,14 +728,34 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_per_launch": bpv * V,
+            "bytes_per_launch": LOD_BYTES_PER_VOXEL * V,
+            "bytes_per_voxel": LOD_BYTES_PER_VOXEL,
+            "bytes_per_voxel_variant_moves": moved,
             "us_per_launch": lod_us,
+        },
+        # the second target kernel: force + movement, 144 B per agent-step
+        "roofline_force": {
+            "bound": "hbm",
+            "kernel": f"force + movement (fm_force_move, variant {tuned.get('fm_prefetch')})",
+            "achieved": force_achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": force_achieved / HBM_PEAK_GBS,
+            "traffic": force_traffic,
+            "bytes_per_launch": FORCE_BYTES_PER_CELL * g["n_prof"],
+            "us_per_launch": force_us,
         },
         "cpu_baseline": cpu,
         "phase_us": phases,
         "kernel_variants": g["tuned"],
         "device": g["device"],
     }
+    if args.config == "L" and args.surrogate:
+        # the metric's third clause: the surrogate against plain LOD on the
+        # same workload (oxygen only), one biology step each
+        log("[bench] surrogate leg")
+        sl = surrogate_leg(args)
+        line["surrogate"] = {k: sl[k] for k in SURROGATE_SUMMARY_KEYS if k in sl}
     print(json.dumps(line), flush=True)
     return 0
 
