@@ -519,6 +519,8 @@ int gell_device_name(char* buf, int len) {
 
 namespace gell {
 
+int fail_global(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+
 bool substrate_stream_on(const gell_ctx* ctx, int s) {
     return s >= 0 && ctx->sub_stream[s] && ctx->tune[GELL_KNOB_SUBSTRATE_STREAMS] && !ctx->prof;
 }

@@ -107,6 +107,10 @@ namespace gell {
 
 enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, PH_LOD, PH_BIRTH, PH_DEATH };
 
+// Records the message of a failed context-free call (read back by
+// gell_last_error(NULL)) and returns `code`.
+int fail_global(int code, const std::string& msg);
+
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);
 hipError_t xspan_rebuild(gell_ctx* ctx, int s, hipStream_t st);

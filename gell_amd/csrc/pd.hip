@@ -9,6 +9,8 @@
 // divisions beyond capacity - 1 are dropped in that same order.
 // Death: stable stream compaction of live records by the same scan.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -280,3 +282,87 @@ hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_o
 }
 
 }  // namespace gell
+
+// ---------------------------------------------------------------------------
+// Death without allocation (gell_death_noalloc): the same four kernels as
+// gell_death, with their scratch (per-block counts and offsets, the live
+// count, the compacted records) in one file-scope device array instead of a
+// context's buffers. Each device holds its own copy of the array, at its own
+// address, until the process ends; host code uses only the address
+// hipGetSymbolAddress returns for the current device, never the array's name.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int32_t kNaMaxCells = GELL_DEATH_NOALLOC_MAX_CELLS;
+constexpr int32_t kNaMaxBlocks = (kNaMaxCells + gell::kBlock - 1) / gell::kBlock;
+constexpr size_t kNaBlkOff = 0;
+constexpr size_t kNaTotalOff = ((sizeof(int32_t) * (size_t)kNaMaxBlocks + 255) / 256) * 256;
+constexpr size_t kNaCellsOff = kNaTotalOff + 256;
+constexpr size_t kNaBytes = kNaCellsOff + sizeof(gell_cell) * (size_t)kNaMaxCells;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) on its first call.
+hipError_t noalloc_scratch(unsigned char** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNaMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_na_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_na_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int gell_death_noalloc(gell_cell* d_cells, int32_t n, void* stream, int32_t* n_out) {
+    using namespace gell;
+    if (!n_out) return fail_global(GELL_E_ARG, "gell_death_noalloc: null n_out");
+    if (n < 0 || (n > 0 && !d_cells)) return fail_global(GELL_E_ARG, "gell_death_noalloc: bad cell array or count");
+    if (n > kNaMaxCells)
+        return fail_global(GELL_E_ARG,
+                           "gell_death_noalloc: n exceeds GELL_DEATH_NOALLOC_MAX_CELLS (use gell_death for more cells)");
+    *n_out = n;
+    if (n == 0) return GELL_OK;
+    unsigned char* base = nullptr;
+    hipError_t e = noalloc_scratch(&base);
+    if (e != hipSuccess) return fail_global(GELL_E_HIP, std::string("gell_death_noalloc: scratch address: ") +
+                                                            hipGetErrorString(e));
+    int32_t* blk = reinterpret_cast<int32_t*>(base + kNaBlkOff);
+    int32_t* total = reinterpret_cast<int32_t*>(base + kNaTotalOff);
+    gell_cell* compact = reinterpret_cast<gell_cell*>(base + kNaCellsOff);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    // every word the kernels read is written earlier in the same call: the
+    // counts by death_count, offsets and total by scan_small, the records by
+    // death_scatter; nothing has to start at zero
+    hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk);
+    hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanThreads), 0, st, blk, (int32_t)blocks, total);
+    hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk, compact);
+    const int64_t words = (int64_t)n * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
+    const int64_t cblocks = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(copy_counted, dim3((unsigned)cblocks), dim3(kBlock), 0, st,
+                       reinterpret_cast<const uint2*>(compact), reinterpret_cast<uint2*>(d_cells), total);
+    e = hipGetLastError();
+    int32_t live = n;
+    if (e == hipSuccess) e = hipMemcpyAsync(&live, total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail_global(GELL_E_HIP, std::string("gell_death_noalloc: ") + hipGetErrorString(e));
+    *n_out = live;
+    return GELL_OK;
+}

@@ -113,6 +113,7 @@ def load_library() -> C.CDLL:
         "gell_lod_substep": (C.c_int, [vp, i32]),
         "gell_birth": (C.c_int, [vp, vp, i32, u64, P(i32)]),
         "gell_death": (C.c_int, [vp, vp, i32, P(i32)]),
+        "gell_death_noalloc": (C.c_int, [vp, i32, vp, P(i32)]),
         "gell_field": (vp, [vp, i32]),
         "gell_uptake_field": (vp, [vp, i32]),
         "gell_sorted_keys": (vp, [vp]),

@@ -138,6 +138,22 @@ int gell_birth(gell_ctx* ctx, gell_cell* d_cells, int32_t n, uint64_t bio_step, 
 /* Stable removal of records with sign == 0 from d_cells[0, n). Synchronises. */
 int gell_death(gell_ctx* ctx, gell_cell* d_cells, int32_t n, int32_t* n_out);
 
+/* gell_death without a context and without allocating: no counterpart in
+ * the reference. It is for callers that cannot allocate device memory (code
+ * running while a stream is captured, tight serving loops, harnesses that
+ * forbid hipMalloc): it calls no hipMalloc, hipFree or allocating library
+ * routine. Its scratch (per-block counts, the live count and the compacted
+ * records) is one static device array of the library, allocated with the
+ * code object and held on every device the library runs on until the process
+ * ends (about 72 B x GELL_DEATH_NOALLOC_MAX_CELLS, 9.4 MB). Same kernels and
+ * same result as gell_death; `stream` is a hipStream_t (NULL = null stream);
+ * synchronises it. n > GELL_DEATH_NOALLOC_MAX_CELLS returns GELL_E_ARG
+ * (gell_last_error(NULL) says why). All calls on one device share the
+ * array: calls that overlap on one device (other host threads, other
+ * streams) race, so callers serialise them. */
+#define GELL_DEATH_NOALLOC_MAX_CELLS 131072
+int gell_death_noalloc(gell_cell* d_cells, int32_t n, void* stream, int32_t* n_out);
+
 /* Device pointers owned by the context (valid until gell_destroy). Both
  * first order the substrate's own stream (GELL_KNOB_SUBSTRATE_STREAMS) back
  * into the context stream, so work enqueued afterwards on the context stream
