@@ -908,6 +908,109 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_staged(const gell_cell* 
         reinterpret_cast<uint2*>(cells + b0)[2 * nq] = make_uint2(rec[4 * nq], rec[4 * nq + 1]);
 }
 
+// Workgroup -> tile order for the eight XCDs: the hardware hands workgroup i
+// to XCD i mod 8, and each XCD has its own L2. Tile = XCD x's share in one
+// contiguous run of the sorted cells (a contiguous stretch of the Morton
+// curve), so the workgroups an XCD runs at the same time are spatial
+// neighbours and share their stencils' ranges and candidates in that L2.
+// A bijection of [0, G).
+__device__ __forceinline__ int xcd_tile(int i, int G) {
+    const int x = i & 7, q = i >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+// Per-substrate uptake accumulation fused into the mechanics step (PreLOD's
+// oxy_consumption_update, LODkernel.cuh:12-22, on the moved cells): each live
+// cell inside the domain adds its Cell::O2_consume() rate of every substrate,
+// as 2^-32 fixed point, to the substrate's int64 accumulator at its new
+// mesh_idx, and its voxel goes to `list` (-1 for none).
+struct UptakeArgs {
+    int32_t S;  // substrates to accumulate (0: not fused)
+    long long* acc[GELL_MAX_SUBSTRATES];
+    float uptake[GELL_MAX_SUBSTRATES];
+    float dead_factor[GELL_MAX_SUBSTRATES];
+    int32_t* list;
+};
+
+__device__ __forceinline__ void accumulate_uptake(const CellRegs& c, int64_t slot, int64_t V, float dx,
+                                                  const UptakeArgs& ua) {
+    const int32_t v = (int32_t)c.w[W_MESH];
+    const bool take = csign(c) && v >= 0 && v < V;
+    if (take) {
+        for (int s = 0; s < ua.S; ++s) {
+            const float rate = cell_uptake(cf(c, W_V), (int32_t)c.w[W_PHASE], dx, ua.uptake[s], ua.dead_factor[s]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(ua.acc[s] + v), (unsigned long long)to_fixed(rate));
+        }
+    }
+    ua.list[slot] = take ? v : -1;
+}
+
+// Variants 7 and 8: the occupancy walk of variant 6 on XCD-ordered tiles.
+// STAGED (7): the tile's records are read and written as coalesced 16-byte
+// chunks through LDS (as variant 4) instead of 8-byte per-lane accesses.
+// With ua.S > 0 the moved cells' uptake rates are accumulated as well.
+// Same force / movement bits as variants 0-6.
+template <bool STAGED>
+__global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __restrict__ sorted,
+                                                            const float4* __restrict__ nb,
+                                                            const int2* __restrict__ se,
+                                                            gell_cell* __restrict__ cells, int32_t n, int32_t N,
+                                                            float dx, float dt, const uint32_t* __restrict__ occ,
+                                                            UptakeArgs ua) {
+    __shared__ uint32_t rec[STAGED ? kBlock * kRecDw : 1];
+    const int64_t b0 = (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * kBlock;
+    const int64_t i = b0 + threadIdx.x;
+    const bool in_range = i < n;
+    const float4 me = in_range ? load_nb(nb_rsrc(nb), (int32_t)i) : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
+    const bool live = __float_as_uint(me.w) != kDeadTag;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (live) cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+    const int64_t V = (int64_t)N * N * N;
+    if (!STAGED) {
+        if (!in_range) return;
+        CellRegs c = load_cell(sorted, i);
+        if (live) move_cell(c, fx, fy, fz, N, dx, dt);
+        store_cell(cells, i, c);
+        if (ua.S > 0) accumulate_uptake(c, i, V, dx, ua);
+        return;
+    }
+    // the tile's records, coalesced, into LDS (after the walk, so that the
+    // walk's registers are free again)
+    const int m = (int)((int64_t)n - b0 < kBlock ? (int64_t)n - b0 : kBlock);
+    const int ndw = m * kRecDw;
+    const int nq = ndw / 4;  // whole 16-byte chunks; an odd m leaves one 8-byte tail
+    const uint4* src = reinterpret_cast<const uint4*>(sorted + b0);
+    for (int q = threadIdx.x; q < nq; q += kBlock) {
+        const uint4 v = src[q];
+        rec[4 * q] = v.x;
+        rec[4 * q + 1] = v.y;
+        rec[4 * q + 2] = v.z;
+        rec[4 * q + 3] = v.w;
+    }
+    if (threadIdx.x == 0 && (ndw & 3)) {
+        const uint2 v = reinterpret_cast<const uint2*>(sorted + b0)[2 * nq];
+        rec[4 * nq] = v.x;
+        rec[4 * nq + 1] = v.y;
+    }
+    __syncthreads();
+    if (in_range) {
+        CellRegs c;
+#pragma unroll
+        for (int w = 0; w < kRecDw; ++w) c.w[w] = rec[threadIdx.x * kRecDw + w];
+        if (live) move_cell(c, fx, fy, fz, N, dx, dt);
+#pragma unroll
+        for (int w = 0; w < kRecDw; ++w) rec[threadIdx.x * kRecDw + w] = c.w[w];
+        if (ua.S > 0) accumulate_uptake(c, i, V, dx, ua);
+    }
+    __syncthreads();
+    uint4* dst = reinterpret_cast<uint4*>(cells + b0);
+    for (int q = threadIdx.x; q < nq; q += kBlock)
+        dst[q] = make_uint4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
+    if (threadIdx.x == 0 && (ndw & 3))
+        reinterpret_cast<uint2*>(cells + b0)[2 * nq] = make_uint2(rec[4 * nq], rec[4 * nq + 1]);
+}
+
 __global__ void blk_reset(int2* __restrict__ blk, int64_t nblk, int32_t* __restrict__ n_out) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b < nblk) blk[b] = make_int2(0x7FFFFFFF, -1);
@@ -1099,7 +1202,19 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
                            ctx->P.dt_mech);
         return hipGetLastError();
     }
-    if (v == 4 && ((reinterpret_cast<uintptr_t>(cells) | reinterpret_cast<uintptr_t>(ctx->sorted)) & 15) != 0) v = 3;
+    const bool aligned16 = ((reinterpret_cast<uintptr_t>(cells) | reinterpret_cast<uintptr_t>(ctx->sorted)) & 15) == 0;
+    if (v == 4 && !aligned16) v = 3;
+    if (v == 7 && !aligned16) v = 8;
+    if (v == 7 || v == 8) {
+        UptakeArgs ua{};
+        if (v == 7)
+            hipLaunchKernelGGL(fm_force_move_xcd<true>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ua);
+        else
+            hipLaunchKernelGGL(fm_force_move_xcd<false>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ua);
+        return hipGetLastError();
+    }
     if (v == 4)
         hipLaunchKernelGGL(fm_force_move_staged, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
                            cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);

@@ -209,8 +209,15 @@ int gell_sync(gell_ctx* ctx);
  *                              first from a one-bit-per-voxel occupancy map
  *                              (Morton-indexed: at most eight 8-byte reads),
  *                              only their ranges loaded, each while the
- *                              previous voxel is walked.
- *                              Default 6.
+ *                              previous voxel is walked,
+ *                          7 = as 6, with workgroups ordered so that each of
+ *                              the eight XCDs (each with its own L2) walks
+ *                              one contiguous run of the sorted cells, and
+ *                              the records read and written as coalesced
+ *                              16-byte chunks staged in LDS (needs 16-byte
+ *                              aligned cell arrays; 8 otherwise),
+ *                          8 = as 7 with per-lane record accesses.
+ *                              Default 7.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4, 8 or 16 = each line split into

@@ -134,7 +134,7 @@ def test_fm_carried_threshold_unequal_radii(oracle):
     oracle.fm_keys(P, sorted_in, n)
     hand = carry_expected(sorted_in, CARRY_ACCEPTED)
     st = GpuState(P, cells)
-    for v in range(7):
+    for v in range(9):
         log(f"[fm] carried thres, force variant {v}")
         st.ctx.tune("fm_prefetch", v)
         st.upload(cells)
