@@ -102,11 +102,11 @@ def read_traffic(name: str):
     return None
 
 
-def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
+def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None, fused=True):
     """Timed biology steps of the workload. With graph, the steps run through
-    the library's graph path (a one-replicate ensemble: each mechanics step
-    after the first of a biology step is one hipGraph launch); otherwise
-    through the per-step calls."""
+    the library's graph path (a one-replicate ensemble); otherwise through the
+    per-step calls, each mechanics step with its uptake maps as one
+    gell_mech_step (fused) or as gell_fm_update + gell_prelod per substrate."""
     from gell_amd import runtime as rt
 
     n = cfg["cells"]
@@ -127,6 +127,14 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
     # reads only its own field and uptake map), so running all substeps of
     # one substrate before the next gives the same fields as the reference's
     # interleaved order while keeping one field's working set in cache.
+    def mech_and_maps(n):
+        if fused:
+            ctx.mech_step(buf, n)
+        else:
+            ctx.fm_update(buf, n)
+            for s in range(P.n_substrates):
+                ctx.prelod(s, buf, n)
+
     def bio_step(b, n):
         if graph:
             agents = n * mech  # counts are fixed between birth and death
@@ -135,9 +143,7 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
         n = ctx.birth(buf, n, b)
         agents = 0
         for _ in range(mech):
-            ctx.fm_update(buf, n)
-            for s in range(P.n_substrates):
-                ctx.prelod(s, buf, n)
+            mech_and_maps(n)
             for s in range(P.n_substrates):
                 for _ in range(diff):
                     ctx.lod_substep(s)
@@ -160,22 +166,29 @@ def gpu_leg(cfg, P, cells, steps, warmup, graph=False, knobs=None):
     # Per-phase device times come from a separate, untimed pass with the
     # library's event profiling on (which also runs every substrate on the
     # context stream, so each substep is timed alone): mechanics steps of the
-    # same state, then one more biology step for birth and death.
+    # same state as timed, then one more biology step for birth and death;
+    # then mechanics steps with separate uptake maps, which time the force +
+    # movement kernel alone.
     ctx.profile(True)
     ctx.profile_read(reset=True)
     n_prof = n  # cells of each profiled mechanics step
     for _ in range(min(mech, 10)):
-        ctx.fm_update(buf, n)
-        for s in range(P.n_substrates):
-            ctx.prelod(s, buf, n)
+        mech_and_maps(n)
         for s in range(P.n_substrates):
             for _ in range(diff):
                 ctx.lod_substep(s)
     n = ctx.death(buf, ctx.birth(buf, n, warmup + steps))
     prof = ctx.profile_read(reset=True)
+    n_alone = n
+    for _ in range(5):
+        ctx.fm_update(buf, n)
+        for s in range(P.n_substrates):
+            ctx.prelod(s, buf, n)
+    prof_alone = ctx.profile_read(reset=True)
     ctx.profile(False)
     out = dict(seconds=t1 - t0, agents=agents, n_final=n, mech=mech, diff=diff, prof=prof,
-               device=rt.device_name(), tuned=tuned, n_prof=n_prof)
+               device=rt.device_name(), tuned=tuned, n_prof=n_prof, prof_alone=prof_alone, n_alone=n_alone,
+               fused=fused)
     buf.free()
     (ens if graph else ctx).close()
     return out
@@ -220,6 +233,9 @@ def cpu_leg(cfg, P, cells, budget_s):
 def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | None = None) -> dict:
     """One kernel group alone, for profiler runs: `lod` (LOD substeps of
     substrate 0 after one mechanics step + uptake map), `fm` (mechanics steps),
+    `mech` (mechanics steps with the uptake maps accumulated by the force
+    kernel, gell_mech_step), `mech_unfused` (mechanics step + one uptake map
+    per substrate, the calls gell_mech_step stands for),
     `surrogate` (predictions of the committed LOD surrogate on that field and
     uptake map) or `copy` (the streaming copy kernel over 256 MiB, the
     byte-counter calibration)."""
@@ -274,10 +290,17 @@ def only_leg(cfg, P, cells, what: str, steps: int, warmup: int, knobs: dict | No
                 ctx.profile_read(reset=True)
             if what == "lod":
                 ctx.lod_substep(0)
+            elif what == "mech":
+                ctx.mech_step(buf, n)
+            elif what == "mech_unfused":
+                ctx.fm_update(buf, n)
+                for s in range(P.n_substrates):
+                    ctx.prelod(s, buf, n)
             else:
                 ctx.fm_update(buf, n)
         prof = ctx.profile_read(reset=True)
         out["phase_us"] = {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items() if v[1]}
+        out["phase_calls"] = {k: v[1] for k, v in prof.items() if v[1]}
     buf.free()
     ctx.close()
     return out
@@ -627,6 +650,8 @@ def main() -> int:
     ap.add_argument("--weights", default=str(SURROGATE_WEIGHTS), help="surrogate parameters (config N)")
     ap.add_argument("--graph", action="store_true", help="time through the graph path (one-replicate ensemble)")
     ap.add_argument("--sur-tol", type=float, default=1e-3, help="surrogate acceptance tolerance (config N)")
+    ap.add_argument("--unfused", dest="fused", action="store_false",
+                    help="mechanics step and uptake maps as separate calls (gell_fm_update + gell_prelod)")
     ap.add_argument("--no-surrogate", dest="surrogate", action="store_false",
                     help="config L: skip the surrogate-vs-LOD leg")
     ap.add_argument("--drift-steps", type=int, default=0,
@@ -634,7 +659,7 @@ def main() -> int:
                          "and FP64 oracles (config L or S)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-oracle sample budget (0 = skip)")
-    ap.add_argument("--only", choices=["lod", "fm", "copy", "surrogate"], default=None,
+    ap.add_argument("--only", choices=["lod", "fm", "mech", "mech_unfused", "copy", "surrogate"], default=None,
                     help="run one kernel group alone (profiler runs); prints its timings")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="kernel-variant knob (e.g. lod_parts=8); in a full run it overrides the autotuner's choice")
@@ -659,7 +684,8 @@ def main() -> int:
     if args.only:
         print(json.dumps(only_leg(cfg, P, cells, args.only, args.steps, args.warmup, knobs)), flush=True)
         return 0
-    g = gpu_leg(cfg, P, cells, args.steps, args.warmup, graph=cfg.get("graph", False) or args.graph, knobs=knobs)
+    g = gpu_leg(cfg, P, cells, args.steps, args.warmup, graph=cfg.get("graph", False) or args.graph, knobs=knobs,
+                fused=args.fused)
 
     secs = g["seconds"]
     V = cfg["N"] ** 3
@@ -672,9 +698,9 @@ def main() -> int:
     achieved = LOD_BYTES_PER_VOXEL * V / (lod_us * 1e-6) / 1e9
     traffic = read_traffic(f"lod_substep_{cfg['N']}_traffic.txt")
     phases = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in g["prof"].items() if v[1]}
-    force_ms, force_calls = g["prof"]["fm_force_move"]
+    force_ms, force_calls = g["prof_alone"]["fm_force_move"]
     force_us = force_ms / max(force_calls, 1) * 1e3
-    force_achieved = FORCE_BYTES_PER_CELL * g["n_prof"] / (force_us * 1e-6) / 1e9
+    force_achieved = FORCE_BYTES_PER_CELL * g["n_alone"] / (force_us * 1e-6) / 1e9
     force_traffic = read_traffic(f"fm_{cfg['N']}_traffic.txt")
 
     cpu = None
@@ -712,7 +738,9 @@ def main() -> int:
                          f"{g['mech']} x (FM + uptake maps + {g['diff']} LOD substeps per substrate) + death"),
             "grid": cfg["N"],
             "cells": cfg["cells"],
-            "path": "graph (one-replicate ensemble)" if (cfg.get("graph", False) or args.graph) else "per-step calls",
+            "path": ("graph (one-replicate ensemble)" if (cfg.get("graph", False) or args.graph) else
+                     "per-step calls, " + ("gell_mech_step (uptake maps accumulated by the force kernel)" if
+                                           args.fused else "gell_fm_update + gell_prelod")),
             "substrates": P.n_substrates,
             "mech_steps_per_step": g["mech"],
             "lod_substeps_per_mech_step": g["diff"],
@@ -742,8 +770,10 @@ def main() -> int:
             "unit": "GB/s",
             "frac": force_achieved / HBM_PEAK_GBS,
             "traffic": force_traffic,
-            "bytes_per_launch": FORCE_BYTES_PER_CELL * g["n_prof"],
+            "bytes_per_launch": FORCE_BYTES_PER_CELL * g["n_alone"],
             "us_per_launch": force_us,
+            "timed_in": "mechanics steps with separate uptake maps (the kernel alone)",
+            "fused_us_per_launch": (phases.get("fm_force_move") if g["fused"] else None),
         },
         "cpu_baseline": cpu,
         "phase_us": phases,

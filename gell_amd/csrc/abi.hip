@@ -111,6 +111,9 @@ void release(gell_ctx* c) {
     dfree(c->se_list);
     dfree(c->mblk);
     dfree(c->occ);
+    dfree(c->occr);
+    dfree(c->cs);
+    dfree(c->rank_tot);
     dfree(c->n_out);
     for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) {
         dfree(c->field[s]);
@@ -131,6 +134,8 @@ void release(gell_ctx* c) {
     dfree(c->slots);
     dfree(c->d_total);
     dfree(c->compact);
+    dfree(c->mlist[0]);
+    dfree(c->mlist[1]);
     if (c->h_total) (void)hipHostFree(c->h_total);
 }
 
@@ -223,6 +228,11 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->mblk, (size_t)c->nmblk));
     c->occ_words = c->KS >= 64 ? c->KS / 32 : 2;
     A(dalloc(&c->occ, (size_t)c->occ_words));
+    c->occr_words = c->KS >= 64 ? c->KS / 64 : 1;
+    A(dalloc(&c->occr, (size_t)c->occr_words));
+    A(dalloc(&c->cs, cap + 2));
+    c->rank_blocks = (c->occr_words + 4095) / 4096;
+    A(dalloc(&c->rank_tot, (size_t)c->rank_blocks));
     A(dalloc(&c->n_out, 1));
     for (int s = 0; s < P.n_substrates; ++s) {
         A(dalloc(&c->field[s], (size_t)c->V));
@@ -233,10 +243,13 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
         A(dalloc(&c->ptab[s][1], (size_t)(4 * P.N + 64)));
         A(dalloc(&c->ptab[s][2], (size_t)(4 * P.N + 256)));
         A(dalloc(&c->ptab[s][3], (size_t)(4 * P.N + 1024)));
-        A(dalloc(&c->acc[s], (size_t)c->V));
+        A(dalloc(&c->acc[s], (size_t)c->KS));  // Morton-indexed
         A(dalloc(&c->rate_list[s], cap));
+        c->rate_list_ptr[s] = c->rate_list[s];
         A(dalloc(&c->xspan[s], (size_t)P.N * P.N));
     }
+    A(dalloc(&c->mlist[0], cap));
+    A(dalloc(&c->mlist[1], cap));
     A(dalloc(&c->flags, cap));
     A(dalloc(&c->blk, (size_t)c->blk_cap));
     A(dalloc(&c->slots, cap));
@@ -260,7 +273,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     for (int s = 0; s < P.n_substrates && e == hipSuccess; ++s) {
         hipLaunchKernelGGL(fill_f32, dim3(vb), dim3(256), 0, c->stream, c->field[s], c->V, P.sub[s].initial);
         A(hipMemsetAsync(c->rate[s], 0, sizeof(float) * (size_t)c->V, c->stream));
-        A(hipMemsetAsync(c->acc[s], 0, sizeof(long long) * (size_t)c->V, c->stream));
+        A(hipMemsetAsync(c->acc[s], 0, sizeof(long long) * (size_t)c->KS, c->stream));
         A(gell::lod_init_coeffs(c, s));
     }
     A(gell::fm_init_ranges(c));
@@ -283,10 +296,13 @@ void gell_destroy(gell_ctx* ctx) {
     delete ctx;
 }
 
-int gell_fm_update(gell_ctx* ctx, gell_cell* d_cells, int32_t n) {
-    if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
-    int r = check_cells(ctx, d_cells, n);
-    if (r) return r;
+}  // extern "C"
+
+namespace {
+
+// FM_update (FMkernel.cuh:181-202); with uptake_list the force kernel also
+// accumulates the uptake maps (gell_mech_step).
+int fm_step(gell_ctx* ctx, gell_cell* d_cells, int32_t n, int32_t* uptake_list) {
     hipError_t e;
     {
         PhaseScope ps(ctx, gell::PH_FM_KEYS);
@@ -305,10 +321,52 @@ int gell_fm_update(gell_ctx* ctx, gell_cell* d_cells, int32_t n) {
     if (e != hipSuccess) return hip_fail(ctx, e, "fm gather");
     {
         PhaseScope ps(ctx, gell::PH_FM_FORCE);
-        e = gell::fm_force_move_launch(ctx, d_cells, n);
+        e = gell::fm_force_move_launch(ctx, d_cells, n, uptake_list);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "fm force/move");
     return GELL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gell_fm_update(gell_ctx* ctx, gell_cell* d_cells, int32_t n) {
+    if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
+    int r = check_cells(ctx, d_cells, n);
+    if (r) return r;
+    return fm_step(ctx, d_cells, n, nullptr);
+}
+
+int gell_mech_step(gell_ctx* ctx, gell_cell* d_cells, int32_t n) {
+    if (!ctx) return fail(nullptr, GELL_E_ARG, "null context");
+    int r = check_cells(ctx, d_cells, n);
+    if (r) return r;
+    if (!gell::fm_fuses_uptake(ctx)) {
+        // the chosen force variant cannot accumulate: the two calls it stands for
+        r = fm_step(ctx, d_cells, n, nullptr);
+        for (int s = 0; r == GELL_OK && s < ctx->P.n_substrates; ++s) r = gell_prelod(ctx, s, d_cells, n);
+        return r;
+    }
+    int32_t* list = gell::prelod_fused_list(ctx);
+    r = fm_step(ctx, d_cells, n, list);
+    if (r) return r;
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < ctx->P.n_substrates && e == hipSuccess; ++s) {
+        // the substrate's previous substeps read the map about to be rewritten
+        e = gell::join_substrate(ctx, s);
+        if (e != hipSuccess) break;
+        {
+            PhaseScope ps(ctx, gell::PH_PRELOD);
+            e = gell::prelod_fused_finalize(ctx, s, list, n);
+        }
+        if (e == hipSuccess && gell::substrate_stream_on(ctx, s)) {
+            e = hipEventRecord(ctx->sub_fork[s], ctx->stream);
+            ctx->sub_forked[s] = e == hipSuccess;
+        }
+    }
+    if (e == hipSuccess) e = gell::prelod_fused_zero(ctx, list, n);
+    return e == hipSuccess ? GELL_OK : hip_fail(ctx, e, "mech step uptake maps");
 }
 
 int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int32_t n) {
@@ -437,7 +495,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
     const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
-                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 8));
+                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 9));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     const hipError_t e = gell::join_substrates(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "tune");

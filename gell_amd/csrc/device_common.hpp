@@ -148,6 +148,58 @@ __device__ __forceinline__ long long to_fixed(float c) {
 }
 __device__ __forceinline__ float from_fixed(long long q) { return __ll2float_rn(q) * (1.0f / 4294967296.0f); }
 
+// Linear voxel index (x fastest) of a Morton code.
+__device__ __forceinline__ int64_t lin_of_code(uint32_t code, int32_t N) {
+    return lin((int32_t)compact3(code), (int32_t)compact3(code >> 1), (int32_t)compact3(code >> 2), N);
+}
+
+// The uptake accumulators of up to GELL_MAX_SUBSTRATES substrates: int64
+// sums in 2^-32 units, indexed by Morton code (so that cells in sorted order
+// add to neighbouring words).
+struct UptakeSet {
+    int32_t S;
+    long long* acc[GELL_MAX_SUBSTRATES];
+    float uptake[GELL_MAX_SUBSTRATES];
+    float dead_factor[GELL_MAX_SUBSTRATES];
+};
+
+// oxy_consumption_update (LODkernel.cuh:12-22) for one cell per lane; every
+// lane of the wave calls it. A live cell whose mesh_idx v lies in [0, N^3)
+// adds Cell::O2_consume() of every substrate at v, and `out_code` is set to
+// v's Morton code (-1 for a cell that adds nothing). Lanes with the same
+// code next to each other (cells in sorted order) are summed in the wave
+// first, so a run issues one atomic per substrate. Integer sums: the result
+// does not depend on the order or grouping of the additions.
+__device__ __forceinline__ void uptake_accumulate_wave(bool take, int32_t v, float V, int32_t phase, int32_t N,
+                                                       float dx, const UptakeSet& u, int32_t& out_code) {
+    int32_t code = -1;
+    if (take) {
+        const int32_t i = v % N, j = (v / N) % N, k = v / (N * N);
+        code = (int32_t)morton3(i, j, k);
+    }
+    out_code = code;
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t prev = __shfl_up(code, 1, 64), next = __shfl_down(code, 1, 64);
+    const bool head = lane == 0 || prev != code;
+    const uint64_t tails = __ballot(lane == 63 || next != code);
+    // last lane of this lane's run
+    const int last = (int)__builtin_ctzll(tails & ~((1ull << lane) - 1ull));
+    for (int s = 0; s < u.S; ++s) {
+        long long q = take ? to_fixed(cell_uptake(V, phase, dx, u.uptake[s], u.dead_factor[s])) : 0;
+        // inclusive prefix sum over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long y = __shfl_up(q, d, 64);
+            if (lane >= d) q += y;
+        }
+        const long long at_last = __shfl(q, last, 64);
+        const long long before = __shfl_up(q, 1, 64);
+        const long long run = at_last - (lane > 0 ? before : 0);
+        if (head && code >= 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(u.acc[s] + code), (unsigned long long)run);
+    }
+}
+
 struct Bio {
     float V, Vf, Vcs, Vns, r, clock;
     int32_t phase;

@@ -37,16 +37,25 @@ constexpr int kBlock = 256;
 // ---------------------------------------------------------------------------
 // Keys
 // ---------------------------------------------------------------------------
+// 32-bit occupancy word holding Morton code v: word v / 32 of the plain
+// bitmap, or the matching half of the 16-byte (bits, bits, rank, -) record of
+// the rank-compacted list. Non-decreasing in v either way.
+__device__ __forceinline__ uint32_t occ_word32(uint32_t v, bool ranked) {
+    return ranked ? (((v >> 6) << 2) | ((v >> 5) & 1u)) : (v >> 5);
+}
+
 __global__ __launch_bounds__(kBlock) void fm_keys(const gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
                                                   uint32_t* __restrict__ keys, const int32_t* __restrict__ clear_list,
                                                   int32_t clear_n, int2* __restrict__ se, int32_t s_empty,
-                                                  uint32_t* __restrict__ occ) {
+                                                  uint32_t* __restrict__ occ, int ranked) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < clear_n) {
+        // what the previous gather wrote: its ranges (plain list only) and its
+        // occupancy words (the whole word: each voxel set in it is on the list)
         const int32_t v = clear_list[i];
         if (v >= 0) {
-            se[v] = make_int2(s_empty, -1);
-            occ[(uint32_t)v >> 5] = 0u;  // the whole word: each voxel set in it is on the list
+            if (!ranked) se[v] = make_int2(s_empty, -1);
+            occ[occ_word32((uint32_t)v, ranked)] = 0u;
         }
     }
     if (i < n) {
@@ -397,10 +406,12 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
                                                     gell_cell* __restrict__ sorted, float4* __restrict__ nb,
                                                     int2* __restrict__ se, int32_t* __restrict__ se_list,
                                                     const int32_t* __restrict__ skeys, int2* __restrict__ blk,
-                                                    int32_t* __restrict__ n_out, uint32_t* __restrict__ occ) {
+                                                    int32_t* __restrict__ n_out, uint32_t* __restrict__ occ,
+                                                    int ranked) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // Voxel (Morton code; -1 outside, INT32_MAX past n: non-decreasing along
-    // the wave), occupancy word (code / 32) and bit, and the slot if live.
+    // the wave), 32-bit occupancy word and bit, and the slot if live. With
+    // `ranked` (rank-compacted list) se is null: no ranges are kept here.
     int32_t kv = INT32_MAX, ow = INT32_MAX;
     uint32_t obit = 0u;
     int32_t lmin = INT32_MAX, lmax = -1;
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
         se_list[i] = v;
         nb[i] = make_float4(px, py, pz, live ? cf(c, W_R) : __uint_as_float(kDeadTag));
         kv = in ? (int32_t)code : -1;
-        ow = in ? (int32_t)(code >> 5) : -1;
+        ow = in ? (int32_t)occ_word32(code, ranked) : -1;
         obit = live ? 1u << (code & 31u) : 0u;
         lmin = live ? (int32_t)i : INT32_MAX;
         lmax = live ? (int32_t)i : -1;
@@ -459,9 +470,104 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
     }
     const int32_t wprev = __shfl_up(ow, 1), kprev = __shfl_up(kv, 1), knext = __shfl_down(kv, 1);
     // Mech_Mesh_se_Get, FMkernel.cuh:45-53: min/max of the live slots.
-    if (lmin != INT32_MAX && (lane == 0 || kprev != kv)) atomicMin(&se[kv].x, lmin);
-    if (lmax >= 0 && (lane == 63 || knext != kv)) atomicMax(&se[kv].y, lmax);
+    if (se && lmin != INT32_MAX && (lane == 0 || kprev != kv)) atomicMin(&se[kv].x, lmin);
+    if (se && lmax >= 0 && (lane == 63 || knext != kv)) atomicMax(&se[kv].y, lmax);
     if (obit != 0u && (lane == 0 || wprev != ow)) atomicOr(&occ[ow], obit);
+}
+
+// ---------------------------------------------------------------------------
+// Rank-compacted cell list (force variant 9): ranks of the occupied voxels
+// in Morton order, then the first live slot of each.
+// ---------------------------------------------------------------------------
+constexpr int kRankPer = 16;                      // words per thread
+constexpr int kRankWords = kBlock * kRankPer;     // words per workgroup (4096)
+
+__device__ __forceinline__ int occr_count(const uint4* __restrict__ occr, int64_t w, int64_t words) {
+    if (w >= words) return 0;
+    const uint4 r = occr[w];
+    return __popc(r.x) + __popc(r.y);
+}
+
+// Occupied voxels per workgroup's 4096 words (coalesced: thread t reads words
+// t, t + 256, ...).
+__global__ __launch_bounds__(kBlock) void rank_count(const uint4* __restrict__ occr, int64_t words,
+                                                     int32_t* __restrict__ tot) {
+    __shared__ int lds4[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kRankWords;
+    int c = 0;
+#pragma unroll 4
+    for (int k = 0; k < kRankPer; ++k) c += occr_count(occr, base + (int64_t)k * kBlock + threadIdx.x, words);
+    int total;
+    block_exclusive_scan(c, lds4, total);
+    if (threadIdx.x == 0) tot[blockIdx.x] = total;
+}
+
+// Exclusive scan of the per-workgroup counts (one workgroup); the sentinel
+// cs[R] = n after the last occupied voxel.
+__global__ __launch_bounds__(kBlock) void rank_scan(int32_t* __restrict__ tot, int32_t m, int32_t* __restrict__ cs,
+                                                    int32_t n) {
+    __shared__ int lds4[kBlock / 64];
+    int carry = 0;
+    for (int base = 0; base < m; base += kBlock) {
+        const int idx = base + (int)threadIdx.x;
+        const int x = idx < m ? tot[idx] : 0;
+        int total;
+        const int ex = block_exclusive_scan(x, lds4, total);
+        if (idx < m) tot[idx] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) cs[carry] = n;
+}
+
+// Each word's rank base into its record, in the same word order as
+// rank_count; the cs entries of its voxels start at INT32_MAX (fm_cs takes
+// the minimum of the live slots).
+__global__ __launch_bounds__(kBlock) void rank_write(uint4* __restrict__ occr, int64_t words,
+                                                     const int32_t* __restrict__ tot, int32_t* __restrict__ cs) {
+    __shared__ int lds4[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kRankWords;
+    int carry = tot[blockIdx.x];
+    for (int k = 0; k < kRankPer; ++k) {
+        const int64_t w = base + (int64_t)k * kBlock + threadIdx.x;
+        const int c = occr_count(occr, w, words);
+        int total;
+        const int r = carry + block_exclusive_scan(c, lds4, total);
+        if (w < words) {
+            reinterpret_cast<uint32_t*>(occr + w)[2] = (uint32_t)r;
+            for (int j = 0; j < c; ++j) cs[r + j] = INT32_MAX;
+        }
+        carry += total;
+    }
+}
+
+// Rank of occupied voxel `code` from its word's record.
+__device__ __forceinline__ int32_t voxel_rank(uint4 rec, uint32_t code) {
+    const uint64_t bits = ((uint64_t)rec.y << 32) | rec.x;
+    return (int32_t)rec.z + __popcll(bits & ((1ull << (code & 63u)) - 1ull));
+}
+
+// cs[rank(v)] = smallest live slot of voxel v: a segmented minimum over the
+// wave's lanes (sorted slots, so a voxel's lanes are adjacent), one atomicMin
+// per run (runs cut by a wave boundary combine through it).
+__global__ __launch_bounds__(kBlock) void fm_cs(const int32_t* __restrict__ skeys, const float4* __restrict__ nb,
+                                                int32_t n, const uint4* __restrict__ occr,
+                                                int32_t* __restrict__ cs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t kv = INT32_MAX, lmin = INT32_MAX;
+    if (i < n) {
+        kv = skeys[i];
+        const bool live = __float_as_uint(nb[i].w) != kDeadTag;
+        lmin = live ? (int32_t)i : INT32_MAX;
+    }
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d *= 2) {
+        const int32_t k2 = __shfl_down(kv, d), m2 = __shfl_down(lmin, d);
+        if (lane + d < 64 && k2 == kv) lmin = m2 < lmin ? m2 : lmin;
+    }
+    const int32_t kprev = __shfl_up(kv, 1);
+    if (lmin != INT32_MAX && kv >= 0 && (lane == 0 || kprev != kv))
+        atomicMin(&cs[voxel_rank(occr[(uint32_t)kv >> 6], (uint32_t)kv)], lmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -710,7 +816,7 @@ __device__ __forceinline__ StencilCodes stencil_codes(int32_t px, int32_t py, in
 // occupancy bits: one bit per Morton code, so a 64-bit word (code / 64) is an
 // aligned 4 x 4 x 4 block and the stencil touches at most 8 such words.
 __device__ __forceinline__ uint32_t occ_mask(const uint32_t* __restrict__ occ, int32_t N, int32_t px, int32_t py,
-                                             int32_t pz, const StencilCodes& c) {
+                                             int32_t pz, const StencilCodes& c, uint32_t word_bytes = 8) {
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(occ), (short)0, 0x7FFFFFFF, 0x00020000);
     const bool vx[3] = {px > 0, true, px + 1 < N}, vy[3] = {py > 0, true, py + 1 < N}, vz[3] = {pz > 0, true, pz + 1 < N};
@@ -726,7 +832,8 @@ __device__ __forceinline__ uint32_t occ_mask(const uint32_t* __restrict__ occ, i
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const uint32_t wi = ((a ? xe : xs) | (b ? ye : ys) | (d ? ze : zs)) >> 6;
-                const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * 8u), 0, 0));
+                const uint2 w =
+                    __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * word_bytes), 0, 0));
                 W[a][b][d] = ((uint64_t)w.y << 32) | w.x;
             }
     uint32_t mask = 0;
@@ -779,6 +886,60 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
         const int2 nxt = more ? range_of((uint32_t)__builtin_ctz(mask)) : make_int2(0, -1);
         mask &= mask - 1;
         const int e = cur.y;
+        for (; j <= e; ++j) {
+            // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
+            const float4 ahead = load_nb(rs, j < e ? j + 1 : (more ? nxt.x : j));
+            force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
+            o = ahead;
+        }
+        if (!more) break;
+        cur = nxt;
+        j = cur.x;
+    }
+}
+
+// Rank variant of the occupancy walk: the stencil's (at most eight) 16-byte
+// word records give the occupied voxels and their ranks, and voxel q's
+// candidates are the slots [cs[rank], cs[rank + 1]) -- its live cells in
+// ascending order with, at most, dead cells after them (skipped by the dead
+// tag, as the reference skips dead cells inside a range, FMkernel.cuh:86).
+// Visiting order and pair arithmetic as in the other variants.
+__device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, const uint4* __restrict__ occr,
+                                                const int32_t* __restrict__ cs, int32_t N, float dx, float p1x,
+                                                float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
+    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+    const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
+    const __amdgpu_buffer_rsrc_t ro =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(occr), (short)0, 0x7FFFFFFF, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(cs), (short)0, 0x7FFFFFFF, 0x00020000);
+    const StencilCodes sc = stencil_codes(px, py, pz);
+    uint32_t mask = occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16);
+    const auto pick = [](const uint32_t (&v)[3], uint32_t o) { return o == 0 ? v[0] : (o == 1 ? v[1] : v[2]); };
+    if (mask == 0) return;
+    // [first, end) candidate slots of stencil voxel q. Its word's record is
+    // read again (a cache hit: the eight were just loaded) rather than picked
+    // from the registers above, which a runtime index would move to scratch.
+    const auto range_of = [&](uint32_t q) {
+        const uint32_t xo = q / 9, yo = (q / 3) % 3, zo = q % 3;
+        const uint32_t code = pick(sc.cx, xo) | pick(sc.cy, yo) | pick(sc.cz, zo);
+        const uint4 rec =
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ro, (int)((code >> 6) * 16u), 0, 0));
+        const int32_t rk = voxel_rank(rec, code);
+        const int32_t s = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, rk * 4, 0, 0);
+        const int32_t e = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, rk * 4 + 4, 0, 0);
+        return make_int2(s, e);
+    };
+    int2 cur = range_of((uint32_t)__builtin_ctz(mask));
+    mask &= mask - 1;
+    int j = cur.x;
+    float4 o = load_nb(rs, j);
+    while (true) {
+        const bool more = mask != 0;
+        const int2 nxt = more ? range_of((uint32_t)__builtin_ctz(mask)) : make_int2(0, 0);
+        mask &= mask - 1;
+        const int e = cur.y - 1;
         for (; j <= e; ++j) {
             // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
             const float4 ahead = load_nb(rs, j < e ? j + 1 : (more ? nxt.x : j));
@@ -922,42 +1083,35 @@ __device__ __forceinline__ int xcd_tile(int i, int G) {
 
 // Per-substrate uptake accumulation fused into the mechanics step (PreLOD's
 // oxy_consumption_update, LODkernel.cuh:12-22, on the moved cells): each live
-// cell inside the domain adds its Cell::O2_consume() rate of every substrate,
-// as 2^-32 fixed point, to the substrate's int64 accumulator at its new
-// mesh_idx, and its voxel goes to `list` (-1 for none).
-struct UptakeArgs {
-    int32_t S;  // substrates to accumulate (0: not fused)
-    long long* acc[GELL_MAX_SUBSTRATES];
-    float uptake[GELL_MAX_SUBSTRATES];
-    float dead_factor[GELL_MAX_SUBSTRATES];
-    int32_t* list;
-};
-
-__device__ __forceinline__ void accumulate_uptake(const CellRegs& c, int64_t slot, int64_t V, float dx,
-                                                  const UptakeArgs& ua) {
-    const int32_t v = (int32_t)c.w[W_MESH];
-    const bool take = csign(c) && v >= 0 && v < V;
-    if (take) {
-        for (int s = 0; s < ua.S; ++s) {
-            const float rate = cell_uptake(cf(c, W_V), (int32_t)c.w[W_PHASE], dx, ua.uptake[s], ua.dead_factor[s]);
-            atomicAdd(reinterpret_cast<unsigned long long*>(ua.acc[s] + v), (unsigned long long)to_fixed(rate));
-        }
-    }
-    ua.list[slot] = take ? v : -1;
+// cell inside the domain adds its Cell::O2_consume() rate of every substrate
+// to the substrate's Morton-indexed int64 accumulator at its new mesh_idx, and
+// its voxel's Morton code goes to `list` (-1 for none). Every lane of the
+// wave calls it (lanes without a cell take nothing).
+__device__ __forceinline__ void fused_uptake(bool in_range, const CellRegs& c, int64_t slot, int32_t N, float dx,
+                                             const UptakeSet& u, int32_t* __restrict__ list) {
+    const int64_t V = (int64_t)N * N * N;
+    const int32_t v = in_range ? (int32_t)c.w[W_MESH] : -1;
+    const bool take = in_range && csign(c) && v >= 0 && v < V;
+    int32_t code;
+    uptake_accumulate_wave(take, v, cf(c, W_V), (int32_t)c.w[W_PHASE], N, dx, u, code);
+    if (in_range) list[slot] = code;
 }
 
-// Variants 7 and 8: the occupancy walk of variant 6 on XCD-ordered tiles.
-// STAGED (7): the tile's records are read and written as coalesced 16-byte
-// chunks through LDS (as variant 4) instead of 8-byte per-lane accesses.
-// With ua.S > 0 the moved cells' uptake rates are accumulated as well.
-// Same force / movement bits as variants 0-6.
-template <bool STAGED>
+// Variants 7, 8 and 9: the occupancy walk of variant 6 (RANK: the walk of the
+// rank-compacted list, 9) on XCD-ordered tiles. STAGED (7, 9): the tile's
+// records are read and written as coalesced 16-byte chunks through LDS (as
+// variant 4) instead of 8-byte per-lane accesses. With a list the moved
+// cells' uptake rates are accumulated as well. Same force / movement bits as
+// variants 0-6.
+template <bool STAGED, bool RANK>
 __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __restrict__ sorted,
                                                             const float4* __restrict__ nb,
                                                             const int2* __restrict__ se,
                                                             gell_cell* __restrict__ cells, int32_t n, int32_t N,
                                                             float dx, float dt, const uint32_t* __restrict__ occ,
-                                                            UptakeArgs ua) {
+                                                            const uint4* __restrict__ occr,
+                                                            const int32_t* __restrict__ cs, UptakeSet u,
+                                                            int32_t* __restrict__ list) {
     __shared__ uint32_t rec[STAGED ? kBlock * kRecDw : 1];
     const int64_t b0 = (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * kBlock;
     const int64_t i = b0 + threadIdx.x;
@@ -965,14 +1119,20 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
     const float4 me = in_range ? load_nb(nb_rsrc(nb), (int32_t)i) : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
     const bool live = __float_as_uint(me.w) != kDeadTag;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (live) cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
-    const int64_t V = (int64_t)N * N * N;
+    if (live) {
+        if (RANK)
+            cell_force_rank(nb, occr, cs, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+        else
+            cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+    }
     if (!STAGED) {
-        if (!in_range) return;
-        CellRegs c = load_cell(sorted, i);
-        if (live) move_cell(c, fx, fy, fz, N, dx, dt);
-        store_cell(cells, i, c);
-        if (ua.S > 0) accumulate_uptake(c, i, V, dx, ua);
+        CellRegs c{};
+        if (in_range) {
+            c = load_cell(sorted, i);
+            if (live) move_cell(c, fx, fy, fz, N, dx, dt);
+            store_cell(cells, i, c);
+        }
+        if (list) fused_uptake(in_range, c, i, N, dx, u, list);
         return;
     }
     // the tile's records, coalesced, into LDS (after the walk, so that the
@@ -994,15 +1154,15 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
         rec[4 * nq + 1] = v.y;
     }
     __syncthreads();
+    CellRegs c{};
     if (in_range) {
-        CellRegs c;
 #pragma unroll
         for (int w = 0; w < kRecDw; ++w) c.w[w] = rec[threadIdx.x * kRecDw + w];
         if (live) move_cell(c, fx, fy, fz, N, dx, dt);
 #pragma unroll
         for (int w = 0; w < kRecDw; ++w) rec[threadIdx.x * kRecDw + w] = c.w[w];
-        if (ua.S > 0) accumulate_uptake(c, i, V, dx, ua);
     }
+    if (list) fused_uptake(in_range, c, i, N, dx, u, list);
     __syncthreads();
     uint4* dst = reinterpret_cast<uint4*>(cells + b0);
     for (int q = threadIdx.x; q < nq; q += kBlock)
@@ -1108,6 +1268,29 @@ __global__ void se_fill(int2* se, int64_t V, int32_t s_empty) {
     if (v < V) se[v] = make_int2(s_empty, -1);
 }
 
+// Rank-compacted list -> linear (x fastest) ranges of the reference: s is
+// the voxel's first live slot, e the last live slot before the next occupied
+// voxel's first (only dead cells lie in between).
+__global__ void cs_to_linear(const uint4* __restrict__ occr, const int32_t* __restrict__ cs,
+                             const float4* __restrict__ nb, int2* __restrict__ out, int32_t N, int32_t s_empty) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t V = (int64_t)N * N * N;
+    if (v >= V) return;
+    const int32_t i = (int32_t)(v % N), j = (int32_t)((v / N) % N), k = (int32_t)(v / ((int64_t)N * N));
+    const uint32_t code = morton3(i, j, k);
+    const uint4 rec = occr[code >> 6];
+    const uint32_t half = (code & 32u) ? rec.y : rec.x;
+    if (!((half >> (code & 31u)) & 1u)) {
+        out[v] = make_int2(s_empty, -1);
+        return;
+    }
+    const int32_t r = voxel_rank(rec, code);
+    const int32_t s = cs[r];
+    int32_t e = cs[r + 1] - 1;
+    while (e > s && __float_as_uint(nb[e].w) == kDeadTag) --e;
+    out[v] = make_int2(s, e);
+}
+
 // Morton-indexed ranges -> linear (x fastest) layout of the reference.
 __global__ void se_to_linear(const int2* __restrict__ se, int2* __restrict__ out, int32_t N) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1136,22 +1319,30 @@ hipError_t fm_init_ranges(gell_ctx* ctx) {
     hipLaunchKernelGGL(se_fill, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, ctx->KS, ctx->P.capacity + 1);
     hipLaunchKernelGGL(occ_clear, dim3((unsigned)((ctx->occ_words + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        ctx->stream, ctx->occ, ctx->occ_words);
+    hipLaunchKernelGGL(occ_clear, dim3((unsigned)((4 * ctx->occr_words + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, reinterpret_cast<uint32_t*>(ctx->occr), 4 * ctx->occr_words);
     ctx->se_list_n = 0;
+    ctx->cl_mode = 0;
     return hipGetLastError();
 }
 
 hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out) {
     const unsigned blocks = (unsigned)((ctx->V + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(se_to_linear, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, out, ctx->N);
+    if (ctx->cl_mode == 1)
+        hipLaunchKernelGGL(cs_to_linear, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->occr, ctx->cs, ctx->nb, out,
+                           ctx->N, ctx->P.capacity + 1);
+    else
+        hipLaunchKernelGGL(se_to_linear, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->se, out, ctx->N);
     return hipGetLastError();
 }
 
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     const int32_t m = n > ctx->se_list_n ? n : ctx->se_list_n;
     if (m == 0) return hipSuccess;
+    const bool ranked = ctx->cl_mode == 1;  // what the previous gather wrote, cleared here
     hipLaunchKernelGGL(fm_keys, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells, n, ctx->N,
                        ctx->P.dx, ctx->keys[0], ctx->se_list, ctx->se_list_n, ctx->se, ctx->P.capacity + 1,
-                       ctx->occ);
+                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked);
     ctx->se_list_n = 0;
     return hipGetLastError();
 }
@@ -1185,17 +1376,42 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     if (blocks)
         hipLaunchKernelGGL(blk_reset, dim3((unsigned)((ctx->nmblk + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, ctx->mblk, ctx->nmblk, ctx->n_out);
+    const bool ranked = ctx->tune[GELL_KNOB_FM_PREFETCH] == 9;
     hipLaunchKernelGGL(fm_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
-                       ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ctx->se, ctx->se_list,
-                       ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out, ctx->occ);
+                       ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ranked ? nullptr : ctx->se,
+                       ctx->se_list, ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out,
+                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked);
     ctx->se_list_n = n;
+    ctx->cl_mode = ranked ? 1 : 0;
+    if (ranked) {
+        // ranks of the occupied voxels, then each one's first live slot
+        const unsigned rb = (unsigned)ctx->rank_blocks;
+        hipLaunchKernelGGL(rank_count, dim3(rb), dim3(kBlock), 0, ctx->stream, ctx->occr, ctx->occr_words,
+                           ctx->rank_tot);
+        hipLaunchKernelGGL(rank_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->rank_tot, (int32_t)rb, ctx->cs, n);
+        hipLaunchKernelGGL(rank_write, dim3(rb), dim3(kBlock), 0, ctx->stream, ctx->occr, ctx->occr_words,
+                           ctx->rank_tot, ctx->cs);
+        hipLaunchKernelGGL(fm_cs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->out_keys,
+                           ctx->nb, n, ctx->occr, ctx->cs);
+    }
     return hipGetLastError();
 }
 
-hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
+bool fm_fuses_uptake(const gell_ctx* ctx) {
+    const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    return v >= 7 && v <= 9;
+}
+
+hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* uptake_list) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kBlock - 1) / kBlock);
     int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    UptakeSet ua{};
+    if (uptake_list) {
+        // only the XCD-ordered kernels accumulate (fm_fuses_uptake)
+        if (v < 7 || v > 9) return hipErrorInvalidValue;
+        ua = prelod_uptake_set(ctx);
+    }
     if (v == 5) {
         hipLaunchKernelGGL(fm_force_move_blocks, dim3((unsigned)ctx->nmblk), dim3(kBlock), 0, ctx->stream,
                            ctx->sorted, ctx->nb, ctx->se, ctx->mblk, ctx->n_out, cells, ctx->N, ctx->P.dx,
@@ -1205,14 +1421,20 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n) {
     const bool aligned16 = ((reinterpret_cast<uintptr_t>(cells) | reinterpret_cast<uintptr_t>(ctx->sorted)) & 15) == 0;
     if (v == 4 && !aligned16) v = 3;
     if (v == 7 && !aligned16) v = 8;
-    if (v == 7 || v == 8) {
-        UptakeArgs ua{};
+    if (v >= 7 && v <= 9) {
+        const gell_cell* so = ctx->sorted;
         if (v == 7)
-            hipLaunchKernelGGL(fm_force_move_xcd<true>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ua);
+            hipLaunchKernelGGL((fm_force_move_xcd<true, false>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+        else if (v == 8)
+            hipLaunchKernelGGL((fm_force_move_xcd<false, false>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+        else if (aligned16)
+            hipLaunchKernelGGL((fm_force_move_xcd<true, true>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
         else
-            hipLaunchKernelGGL(fm_force_move_xcd<false>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ua);
+            hipLaunchKernelGGL((fm_force_move_xcd<false, true>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
         return hipGetLastError();
     }
     if (v == 4)

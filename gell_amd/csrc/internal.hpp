@@ -40,6 +40,20 @@ struct gell_ctx {
     uint32_t* occ = nullptr;
     int64_t occ_words = 0;
     int32_t se_list_n = 0;
+    // Rank-compacted cell list (force variant 9), instead of se and occ:
+    // per aligned 64-voxel Morton word, (occupancy bits 0-31, bits 32-63, rank
+    // of the word's first occupied voxel among all occupied voxels in Morton
+    // order, unused); cs[r] = smallest live sorted slot of the r-th occupied
+    // voxel, cs[R] = n. The r-th voxel's live cells lie in [cs[r], cs[r+1])
+    // (with only dead cells between them), so the force walk reads dense
+    // arrays (16 B per 64 voxels, 4 B per occupied voxel) instead of 8 B per
+    // voxel of the Morton-indexed ranges.
+    uint4* occr = nullptr;
+    int64_t occr_words = 0;       // max(KS / 64, 1)
+    int32_t* cs = nullptr;        // capacity + 2 entries
+    int32_t* rank_tot = nullptr;  // per-block occupied counts, then their exclusive scan
+    int64_t rank_blocks = 0;
+    int cl_mode = 0;              // what the last gather maintained: 0 se + occ, 1 occr + cs
 
     // Diffusion, per substrate.
     float* field[GELL_MAX_SUBSTRATES] = {};
@@ -55,9 +69,18 @@ struct gell_ctx {
     float* F[GELL_MAX_SUBSTRATES] = {};
     float side[GELL_MAX_SUBSTRATES] = {};
     float* ptab[GELL_MAX_SUBSTRATES][4] = {};  // partitioned-solve tables, P = 2, 4, 8, 16
+    // Uptake accumulators (int64, 2^-32 units), indexed by Morton code (KS
+    // entries); zero except while a map is being built.
     long long* acc[GELL_MAX_SUBSTRATES] = {};
     int32_t* rate_list[GELL_MAX_SUBSTRATES] = {};
     int32_t rate_list_n[GELL_MAX_SUBSTRATES] = {};
+    // The list of voxels (Morton codes, rate_list_n[s] entries) that may hold
+    // a nonzero rate of substrate s: its own rate_list after gell_prelod, one
+    // of the two mlist buffers after a fused mechanics step (gell_mech_step),
+    // whose force kernel writes each moved cell's voxel there while
+    // accumulating every substrate's uptake.
+    const int32_t* rate_list_ptr[GELL_MAX_SUBSTRATES] = {};
+    int32_t* mlist[2] = {nullptr, nullptr};
     // Set when gell_uptake_field handed out the map pointer since the last
     // PreLOD: the caller may have written any voxel, so neither the voxel list
     // nor the x-line spans bound the nonzero rates and accumulators any more,
@@ -128,11 +151,26 @@ hipError_t lod_autotune(gell_ctx* ctx, bool bit_exact, double* best_us);
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
 hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n);
 hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n);
-hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n);
+// With uptake_list the force kernel also accumulates every substrate's uptake
+// rates of the moved cells into acc[] and writes each slot's voxel (Morton
+// code, or -1) to uptake_list; only the variants fm_fuses_uptake() reports
+// can.
+hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* uptake_list = nullptr);
+bool fm_fuses_uptake(const gell_ctx* ctx);
 hipError_t fm_init_ranges(gell_ctx* ctx);
 hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out);
 
 hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n);
+// Fused uptake maps (gell_mech_step): prelod_uptake_set describes every
+// substrate's accumulators for the force kernel; prelod_fused_list picks the
+// list buffer it writes; prelod_fused_finalize clears substrate s's previous
+// map and writes the new one from the accumulators; prelod_fused_zero then
+// zeroes every substrate's accumulators on the list.
+struct UptakeSet;
+UptakeSet prelod_uptake_set(const gell_ctx* ctx);
+int32_t* prelod_fused_list(gell_ctx* ctx);
+hipError_t prelod_fused_finalize(gell_ctx* ctx, int s, const int32_t* list, int32_t n);
+hipError_t prelod_fused_zero(gell_ctx* ctx, const int32_t* list, int32_t n);
 
 hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step, int32_t* n_out);
 hipError_t death_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* n_out);

@@ -109,6 +109,7 @@ def load_library() -> C.CDLL:
         "gell_last_error": (C.c_char_p, [vp]),
         "gell_status_string": (C.c_char_p, [C.c_int]),
         "gell_fm_update": (C.c_int, [vp, vp, i32]),
+        "gell_mech_step": (C.c_int, [vp, vp, i32]),
         "gell_prelod": (C.c_int, [vp, i32, vp, i32]),
         "gell_lod_substep": (C.c_int, [vp, i32]),
         "gell_birth": (C.c_int, [vp, vp, i32, u64, P(i32)]),
@@ -313,6 +314,11 @@ class Context:
     # -- the five operations -------------------------------------------------
     def fm_update(self, cells: DeviceBuffer | int, n: int) -> None:
         self._check(self.lib.gell_fm_update(self.h, _ptr(cells), n), "gell_fm_update")
+
+    def mech_step(self, cells: DeviceBuffer | int, n: int) -> None:
+        """fm_update followed by prelod of every substrate, the uptake rates
+        accumulated by the mechanics step's own force kernel."""
+        self._check(self.lib.gell_mech_step(self.h, _ptr(cells), n), "gell_mech_step")
 
     def prelod(self, s: int, cells: DeviceBuffer | int, n: int) -> None:
         self._check(self.lib.gell_prelod(self.h, s, _ptr(cells), n), "gell_prelod")

@@ -127,6 +127,17 @@ int gell_fm_update(gell_ctx* ctx, gell_cell* d_cells, int32_t n);
  * live cells' uptake rates, taken at mesh_idx). Asynchronous. */
 int gell_prelod(gell_ctx* ctx, int32_t substrate, const gell_cell* d_cells, int32_t n);
 
+/* The mechanics step followed by the uptake map of every substrate: the
+ * sequence FM_update, then PreLOD per substrate, that Main.cu:115-120 runs
+ * (FMkernel.cuh:181-202, LODkernel.cuh:270-278). The force/movement kernel
+ * adds each moved cell's uptake rates to the maps' integer accumulators
+ * while it holds the record, so the records are not read a second time.
+ * Results equal gell_fm_update followed by gell_prelod for substrates
+ * 0 .. n_substrates-1, bit for bit (the integer sums do not depend on
+ * order). With a force variant that cannot accumulate (GELL_KNOB_FM_PREFETCH
+ * below 7) it makes exactly those calls. Asynchronous. */
+int gell_mech_step(gell_ctx* ctx, gell_cell* d_cells, int32_t n);
+
 /* One diffusion substep of one substrate: double-layer Dirichlet boundary,
  * implicit uptake, LOD x/y/z Thomas sweeps. Asynchronous. */
 int gell_lod_substep(gell_ctx* ctx, int32_t substrate);
@@ -216,7 +227,15 @@ int gell_sync(gell_ctx* ctx);
  *                              the records read and written as coalesced
  *                              16-byte chunks staged in LDS (needs 16-byte
  *                              aligned cell arrays; 8 otherwise),
- *                          8 = as 7 with per-lane record accesses.
+ *                          8 = as 7 with per-lane record accesses,
+ *                          9 = as 7 on a rank-compacted cell list: per
+ *                              aligned 64-voxel Morton word its occupancy
+ *                              bits and the rank of its first occupied voxel
+ *                              (16 bytes), and per occupied voxel its first
+ *                              live slot (4 bytes), instead of the 8-byte
+ *                              Morton-indexed range of every voxel.
+ *                              Variants 7-9 also accumulate the uptake maps
+ *                              in gell_mech_step.
  *                              Default 7.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical

@@ -134,7 +134,7 @@ def test_fm_carried_threshold_unequal_radii(oracle):
     oracle.fm_keys(P, sorted_in, n)
     hand = carry_expected(sorted_in, CARRY_ACCEPTED)
     st = GpuState(P, cells)
-    for v in range(9):
+    for v in range(10):
         log(f"[fm] carried thres, force variant {v}")
         st.ctx.tune("fm_prefetch", v)
         st.upload(cells)
@@ -144,4 +144,21 @@ def test_fm_carried_threshold_unequal_radii(oracle):
         f = got["oldforce"][:n].astype(np.float64)
         for i in range(n):
             assert np.linalg.norm(f[i] - hand[i]) <= 2e-5 * np.linalg.norm(hand[i]), (v, i)
+    st.close()
+
+
+@pytest.mark.parametrize("N,n,dense", [(13, 3000, True), (30, 3000, True), (64, 16385, False), (96, 20000, True)])
+def test_fm_rank_list_matches_oracle(N, n, dense, oracle):
+    """Force variant 9 keeps the rank-compacted cell list instead of the
+    Morton-indexed ranges; gell_voxel_ranges rebuilds the reference's s/e
+    from it. Keys, order, ranges and records against the oracle, with the
+    variant switched to 6 and back between steps (each clears what the other
+    left)."""
+    P = params(N, capacity=n + 64)
+    cells = mixed_cells(N, 10.0, n, P.capacity, seed=N + n + 1, dense=dense)
+    st = GpuState(P, cells)
+    state = cells
+    for step, v in enumerate((9, 9, 6, 9)):
+        st.ctx.tune("fm_prefetch", v)
+        state = _check_step(st, state, n, oracle, f"N={N} variant {v} step {step}")
     st.close()

@@ -42,7 +42,7 @@ def test_fm_variants_identical(N, n):
 
 def _all_fm_variants_equal(P, cells, n, steps=3):
     outs = []
-    for v in range(9):
+    for v in range(10):
         st = GpuState(P, cells)
         st.ctx.tune("fm_prefetch", v)
         for _ in range(steps):
@@ -75,7 +75,7 @@ def test_bad_knob_is_an_error():
     with pytest.raises(rt.GellError):
         st.ctx.tune("lod_yz_lds", 7)
     with pytest.raises(rt.GellError):
-        st.ctx.tune("fm_prefetch", 9)
+        st.ctx.tune("fm_prefetch", 10)
     with pytest.raises(KeyError):
         st.ctx.tune("nope", 1)
     st.close()
