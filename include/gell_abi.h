@@ -236,7 +236,7 @@ int gell_sync(gell_ctx* ctx);
  *                              Morton-indexed range of every voxel.
  *                              Variants 7-9 also accumulate the uptake maps
  *                              in gell_mech_step.
- *                              Default 7.
+ *                              Default 9.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4, 8 or 16 = each line split into
