@@ -133,6 +133,7 @@ void release(gell_ctx* c) {
     dfree(c->blk);
     dfree(c->slots);
     dfree(c->d_total);
+    dfree(c->dn);
     dfree(c->compact);
     dfree(c->mlist[0]);
     dfree(c->mlist[1]);
@@ -254,6 +255,7 @@ int gell_create(const gell_params* params, void* stream, gell_ctx** out) {
     A(dalloc(&c->blk, (size_t)c->blk_cap));
     A(dalloc(&c->slots, cap));
     A(dalloc(&c->d_total, 1));
+    A(dalloc(&c->dn, gell::DN_SIZE));
     A(dalloc(&c->compact, cap));
     if (P.N == 128 || P.N == 256 || P.N == 512)
         for (int s = 0; s < P.n_substrates; ++s) A(dalloc(&c->coup[s], (size_t)(4 * (P.N / 64)) * (size_t)P.N * P.N));

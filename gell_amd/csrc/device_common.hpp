@@ -148,6 +148,12 @@ __device__ __forceinline__ long long to_fixed(float c) {
 }
 __device__ __forceinline__ float from_fixed(long long q) { return __ll2float_rn(q) * (1.0f / 4294967296.0f); }
 
+// Cell counts can live on the device (graph-captured biology steps, where no
+// count comes back to the host between birth and death): a kernel given a
+// non-null count pointer takes its count from there instead of from its
+// launch argument, and its grid then covers the capacity.
+__device__ __forceinline__ int32_t count_of(int32_t n, const int32_t* np) { return np ? *np : n; }
+
 // Linear voxel index (x fastest) of a Morton code.
 __device__ __forceinline__ int64_t lin_of_code(uint32_t code, int32_t N) {
     return lin((int32_t)compact3(code), (int32_t)compact3(code >> 1), (int32_t)compact3(code >> 2), N);

@@ -38,6 +38,16 @@ struct gell_ensemble {
     std::vector<int> g_tune;
     int64_t graph_captures = 0;
     int64_t graph_launches = 0;
+    // One graph per biology step (gell_ensemble_bio_step in graph mode), its
+    // counts device-resident: what it was captured for, and the pinned
+    // exchange area its first and last nodes copy the counts through
+    // (per replicate DN_SIZE ints in, then DN_SIZE out).
+    hipGraph_t bgraph = nullptr;
+    hipGraphExec_t bexec = nullptr;
+    std::vector<gell_cell*> b_cells;
+    std::vector<int> b_tune;
+    std::vector<uint8_t> b_state;
+    int32_t* h_io = nullptr;
     std::string err;
 };
 
@@ -53,6 +63,33 @@ int ens_fail(gell_ensemble* ens, int code, const std::string& msg) {
 
 int ens_hip(gell_ensemble* ens, hipError_t e, const char* where) {
     return ens_fail(ens, GELL_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+void drop_bio_graph(gell_ensemble* ens) {
+    if (ens->bexec) (void)hipGraphExecDestroy(ens->bexec);
+    if (ens->bgraph) (void)hipGraphDestroy(ens->bgraph);
+    ens->bexec = nullptr;
+    ens->bgraph = nullptr;
+    ens->b_cells.clear();
+    ens->b_tune.clear();
+    ens->b_state.clear();
+}
+
+// Host-side context state that decides which kernels a captured step
+// launches (which clear an uptake map takes, which cell-list layout the
+// keys kernel clears). A captured biology step is replayed only from the
+// state it was captured in.
+std::vector<uint8_t> state_snapshot(const gell_ensemble* ens) {
+    std::vector<uint8_t> st;
+    for (const gell_ctx* c : ens->ctx) {
+        st.push_back((uint8_t)c->cl_mode);
+        for (int s = 0; s < c->P.n_substrates; ++s) {
+            st.push_back((uint8_t)c->xspan_valid[s]);
+            st.push_back((uint8_t)c->rate_handed_out[s]);
+            st.push_back((uint8_t)(c->rate_list_ptr[s] == c->rate_list[s]));
+        }
+    }
+    return st;
 }
 
 void drop_graph(gell_ensemble* ens) {
@@ -144,6 +181,97 @@ int capture(gell_ensemble* ens, gell_cell* const* cells, const int32_t* n) {
     return GELL_OK;
 }
 
+// One biology step of every replicate (Main.cu:108-127) captured as one
+// graph with device-resident counts: each replicate's chain on its own
+// stream -- counts in from the pinned area, birth, the mechanics steps
+// (FM update, uptake maps, substeps), death, counts out -- joined on the
+// ensemble stream. No count crosses to the host inside the step, so the
+// graph serves every biology step whatever births and deaths do.
+int capture_bio_step(gell_ensemble* ens, gell_cell* const* cells, const int32_t* n) {
+    drop_bio_graph(ens);
+    const size_t R = ens->ctx.size();
+    ens->b_state = state_snapshot(ens);
+    for (gell_ctx* c : ens->ctx) c->devn = true;
+    hipError_t e = hipStreamBeginCapture(ens->main, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        for (gell_ctx* c : ens->ctx) c->devn = false;
+        return ens_hip(ens, e, "begin capture");
+    }
+    int rc = GELL_OK;
+    e = hipEventRecord(ens->fork, ens->main);
+    for (size_t r = 0; r < R && e == hipSuccess && rc == GELL_OK; ++r) {
+        gell_ctx* c = ens->ctx[r];
+        e = hipStreamWaitEvent(ens->streams[r], ens->fork, 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->dn, ens->h_io + (2 * r) * gell::DN_SIZE, sizeof(int32_t) * gell::DN_SIZE,
+                               hipMemcpyHostToDevice, ens->streams[r]);
+        if (e == hipSuccess) e = gell::join_substrates(c);
+        if (e == hipSuccess) e = gell::birth_begin(c, cells[r], n[r], 0);
+        for (int32_t k = 0; k < ens->mech_steps && e == hipSuccess && rc == GELL_OK; ++k)
+            rc = member_step(ens, (int)r, cells[r], n[r]);
+        if (e == hipSuccess && rc == GELL_OK) e = gell::death_begin(c, cells[r], n[r]);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ens->h_io + (2 * r + 1) * gell::DN_SIZE, c->dn, sizeof(int32_t) * gell::DN_SIZE,
+                               hipMemcpyDeviceToHost, ens->streams[r]);
+        if (e == hipSuccess) e = hipEventRecord(ens->join[r], ens->streams[r]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ens->main, ens->join[r], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(ens->main, &g);
+    for (gell_ctx* c : ens->ctx) c->devn = false;
+    if (rc != GELL_OK || e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        ens->b_state.clear();
+        if (rc != GELL_OK) return rc;
+        return ens_hip(ens, e != hipSuccess ? e : e2, "biology step capture");
+    }
+    ens->bgraph = g;
+    e = hipGraphInstantiate(&ens->bexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_bio_graph(ens);
+        return ens_hip(ens, e, "graph instantiate");
+    }
+    ens->b_cells.assign(cells, cells + R);
+    ens->b_tune = tune_snapshot(ens);
+    ++ens->graph_captures;
+    return GELL_OK;
+}
+
+int bio_step_graph(gell_ensemble* ens, gell_cell* const* cells, int32_t* n, uint64_t bio_step) {
+    const size_t R = ens->ctx.size();
+    for (const gell_ctx* c : ens->ctx)
+        if (c->prof) return ens_fail(ens, GELL_E_ARG, "graph mode needs profiling off on every replicate");
+    // the step's counts in: cells, the two list counts, the biology step
+    for (size_t r = 0; r < R; ++r) {
+        const gell_ctx* c = ens->ctx[r];
+        int32_t* in = ens->h_io + (2 * r) * gell::DN_SIZE;
+        in[gell::DN_CELLS] = n[r];
+        in[gell::DN_LIST] = c->se_list_n;
+        for (int s = 0; s < GELL_MAX_SUBSTRATES; ++s) in[gell::DN_MAP + s] = s < c->P.n_substrates ? c->rate_list_n[s] : 0;
+        in[gell::DN_STEP] = (int32_t)(uint32_t)bio_step;
+        in[gell::DN_STEP + 1] = (int32_t)(uint32_t)(bio_step >> 32);
+    }
+    const bool match = ens->bexec && ens->b_cells == std::vector<gell_cell*>(cells, cells + R) &&
+                       ens->b_tune == tune_snapshot(ens) && ens->b_state == state_snapshot(ens);
+    if (!match) {
+        int rc = capture_bio_step(ens, cells, n);
+        if (rc != GELL_OK) return rc;
+    }
+    hipError_t e = hipGraphLaunch(ens->bexec, ens->main);
+    if (e == hipSuccess) e = hipStreamSynchronize(ens->main);
+    if (e != hipSuccess) return ens_hip(ens, e, "biology step graph");
+    ++ens->graph_launches;
+    // the counts out, back into the host-side state the per-context calls use
+    for (size_t r = 0; r < R; ++r) {
+        gell_ctx* c = ens->ctx[r];
+        const int32_t* out = ens->h_io + (2 * r + 1) * gell::DN_SIZE;
+        n[r] = out[gell::DN_CELLS];
+        c->se_list_n = out[gell::DN_LIST];
+        for (int s = 0; s < c->P.n_substrates; ++s) c->rate_list_n[s] = out[gell::DN_MAP + s];
+    }
+    return GELL_OK;
+}
+
 int check_args(gell_ensemble* ens, gell_cell* const* cells, const int32_t* n) {
     if (!ens) return ens_fail(nullptr, GELL_E_ARG, "null ensemble");
     if (!cells || !n) return ens_fail(ens, GELL_E_ARG, "null cell or count array");
@@ -181,6 +309,14 @@ int gell_ensemble_create(const gell_params* params, int32_t count, void* stream,
         ens->own_main = true;
     }
     ens->mech_steps = gell_float_loop_count(params[0].dt_bio, params[0].dt_mech);
+    e = hipHostMalloc(reinterpret_cast<void**>(&ens->h_io), sizeof(int32_t) * 2 * gell::DN_SIZE * (size_t)count);
+    if (e != hipSuccess) {
+        ens->h_io = nullptr;
+        if (ens->own_main) (void)hipStreamDestroy(ens->main);
+        delete ens;
+        return ens_hip(nullptr, e, "pinned count area");
+    }
+    std::memset(ens->h_io, 0, sizeof(int32_t) * 2 * gell::DN_SIZE * (size_t)count);
     ens->diff_steps = gell_float_loop_count(params[0].dt_mech, params[0].dt_diff);
     e = hipEventCreateWithFlags(&ens->fork, hipEventDisableTiming);
     if (e != hipSuccess) rc = ens_hip(ens, e, "event");
@@ -221,7 +357,9 @@ void gell_ensemble_destroy(gell_ensemble* ens) {
     if (!ens) return;
     for (hipStream_t s : ens->streams) (void)hipStreamSynchronize(s);
     drop_graph(ens);
+    drop_bio_graph(ens);
     for (gell_ctx* c : ens->ctx) gell_destroy(c);
+    if (ens->h_io) (void)hipHostFree(ens->h_io);
     for (hipEvent_t j : ens->join) (void)hipEventDestroy(j);
     for (hipStream_t s : ens->streams) (void)hipStreamDestroy(s);
     if (ens->fork) (void)hipEventDestroy(ens->fork);
@@ -248,7 +386,10 @@ int gell_ensemble_set_mode(gell_ensemble* ens, int32_t mode) {
     if (mode != GELL_ENSEMBLE_STREAMS && mode != GELL_ENSEMBLE_GRAPH)
         return ens_fail(ens, GELL_E_ARG, "unknown ensemble mode");
     ens->mode = mode;
-    if (mode != GELL_ENSEMBLE_GRAPH) drop_graph(ens);
+    if (mode != GELL_ENSEMBLE_GRAPH) {
+        drop_graph(ens);
+        drop_bio_graph(ens);
+    }
     return GELL_OK;
 }
 
@@ -273,6 +414,7 @@ int gell_ensemble_autotune(gell_ensemble* ens, int flags) {
         }
     }
     drop_graph(ens);
+    drop_bio_graph(ens);
     return GELL_OK;
 }
 
@@ -305,6 +447,7 @@ int gell_ensemble_mech_steps(gell_ensemble* ens, gell_cell* const* cells, const 
 int gell_ensemble_bio_step(gell_ensemble* ens, gell_cell* const* cells, int32_t* n, uint64_t bio_step) {
     int rc = check_args(ens, cells, n);
     if (rc != GELL_OK) return rc;
+    if (ens->mode == GELL_ENSEMBLE_GRAPH) return bio_step_graph(ens, cells, n, bio_step);
     const size_t R = ens->ctx.size();
     // everything queued on the ensemble stream before this call comes first
     hipError_t e = hipEventRecord(ens->fork, ens->main);

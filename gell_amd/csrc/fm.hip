@@ -47,7 +47,10 @@ __device__ __forceinline__ uint32_t occ_word32(uint32_t v, bool ranked) {
 __global__ __launch_bounds__(kBlock) void fm_keys(const gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
                                                   uint32_t* __restrict__ keys, const int32_t* __restrict__ clear_list,
                                                   int32_t clear_n, int2* __restrict__ se, int32_t s_empty,
-                                                  uint32_t* __restrict__ occ, int ranked) {
+                                                  uint32_t* __restrict__ occ, int ranked,
+                                                  const int32_t* __restrict__ np, const int32_t* __restrict__ clear_np) {
+    n = count_of(n, np);
+    clear_n = count_of(clear_n, clear_np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < clear_n) {
         // what the previous gather wrote: its ranges (plain list only) and its
@@ -113,7 +116,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
 // ---------------------------------------------------------------------------
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist(const uint32_t* __restrict__ keys, int32_t n, int shift,
-                                                           int32_t* __restrict__ hist, int nblocks) {
+                                                           int32_t* __restrict__ hist, int nblocks,
+                                                           const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     constexpr int BINS = 1 << BITS;
     __shared__ int h[BINS];
     for (int b = threadIdx.x; b < BINS; b += kSortThreads) h[b] = 0;
@@ -166,7 +171,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint32_t* __
                                                               uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                               int32_t* __restrict__ signed_out, int32_t n, int shift,
                                                               const int32_t* __restrict__ hist,
-                                                              const int32_t* __restrict__ tot, int nblocks) {
+                                                              const int32_t* __restrict__ tot, int nblocks,
+                                                              const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     constexpr int BINS = 1 << BITS;
     constexpr int PER = (BINS + kSortThreads - 1) / kSortThreads;
     constexpr int NW = kSortThreads / 64;
@@ -301,7 +308,8 @@ __global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __re
                                                                   uint32_t* __restrict__ k1, int32_t* __restrict__ v1,
                                                                   int32_t* __restrict__ order_out,
                                                                   int32_t* __restrict__ signed_out, int32_t n,
-                                                                  SortPlan plan) {
+                                                                  SortPlan plan, const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     __shared__ int cnt[kSmallWaves][kSmallBins];
     __shared__ int dbase[kSmallBins];
     __shared__ int lds16[kSmallWaves];
@@ -384,13 +392,13 @@ __global__ __launch_bounds__(kSmallThreads) void radix_sort_small(uint32_t* __re
 template <int BITS>
 hipError_t sort_pass(gell_ctx* ctx, const uint32_t* kin, const int32_t* vin, uint32_t* kout, int32_t* vout,
                      int32_t* signed_out, int32_t n, int shift) {
-    const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
+    const int nblocks = (int)((grid_count(ctx, n) + kSortTile - 1) / kSortTile);
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL((radix_hist<BITS>), dim3(nblocks), dim3(kSortThreads), 0, st, kin, n, shift, ctx->hist,
-                       nblocks);
+                       nblocks, dev_count(ctx));
     hipLaunchKernelGGL(radix_colscan, dim3(1 << BITS), dim3(kSortThreads), 0, st, ctx->hist, nblocks, ctx->hist_tot);
     hipLaunchKernelGGL((radix_scatter<BITS>), dim3(nblocks), dim3(kSortThreads), 0, st, kin, vin, kout, vout,
-                       signed_out, n, shift, ctx->hist, ctx->hist_tot, nblocks);
+                       signed_out, n, shift, ctx->hist, ctx->hist_tot, nblocks, dev_count(ctx));
     return hipGetLastError();
 }
 
@@ -407,8 +415,11 @@ __global__ __launch_bounds__(kBlock) void fm_gather(const gell_cell* __restrict_
                                                     int2* __restrict__ se, int32_t* __restrict__ se_list,
                                                     const int32_t* __restrict__ skeys, int2* __restrict__ blk,
                                                     int32_t* __restrict__ n_out, uint32_t* __restrict__ occ,
-                                                    int ranked) {
+                                                    int ranked, const int32_t* __restrict__ np,
+                                                    int32_t* __restrict__ list_np) {
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (list_np && i == 0) *list_np = n;  // the next call's keys kernel clears this many list entries
     // Voxel (Morton code; -1 outside, INT32_MAX past n: non-decreasing along
     // the wave), 32-bit occupancy word and bit, and the slot if live. With
     // `ranked` (rank-compacted list) se is null: no ranges are kept here.
@@ -505,7 +516,8 @@ __global__ __launch_bounds__(kBlock) void rank_count(const uint4* __restrict__ o
 // Exclusive scan of the per-workgroup counts (one workgroup); the sentinel
 // cs[R] = n after the last occupied voxel.
 __global__ __launch_bounds__(kBlock) void rank_scan(int32_t* __restrict__ tot, int32_t m, int32_t* __restrict__ cs,
-                                                    int32_t n) {
+                                                    int32_t n, const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     __shared__ int lds4[kBlock / 64];
     int carry = 0;
     for (int base = 0; base < m; base += kBlock) {
@@ -551,7 +563,8 @@ __device__ __forceinline__ int32_t voxel_rank(uint4 rec, uint32_t code) {
 // per run (runs cut by a wave boundary combine through it).
 __global__ __launch_bounds__(kBlock) void fm_cs(const int32_t* __restrict__ skeys, const float4* __restrict__ nb,
                                                 int32_t n, const uint4* __restrict__ occr,
-                                                int32_t* __restrict__ cs) {
+                                                int32_t* __restrict__ cs, const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int32_t kv = INT32_MAX, lmin = INT32_MAX;
     if (i < n) {
@@ -988,7 +1001,9 @@ template <int VARIANT>
 __global__ __launch_bounds__(kBlock) void fm_force_move(const gell_cell* __restrict__ sorted,
                                                         const float4* __restrict__ nb, const int2* __restrict__ se,
                                                         gell_cell* __restrict__ cells, int32_t n, int32_t N, float dx,
-                                                        float dt, const uint32_t* __restrict__ occ) {
+                                                        float dt, const uint32_t* __restrict__ occ,
+                                                        const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in_range = i < n;
     // The cell's own (x, y, z, r) and liveness come from its 16-byte entry of
@@ -1027,10 +1042,12 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_staged(const gell_cell* 
                                                                const float4* __restrict__ nb,
                                                                const int2* __restrict__ se,
                                                                gell_cell* __restrict__ cells, int32_t n, int32_t N,
-                                                               float dx, float dt) {
+                                                               float dx, float dt, const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     __shared__ uint32_t rec[kBlock * kRecDw];
     const int64_t b0 = (int64_t)blockIdx.x * kBlock;
-    const int m = (int)((int64_t)n - b0 < kBlock ? (int64_t)n - b0 : kBlock);  // records in this block
+    // records in this block (none in the blocks past a device-resident count)
+    const int m = (int)max<int64_t>(0, min<int64_t>((int64_t)n - b0, kBlock));
     const int ndw = m * kRecDw;
     const int nq = ndw / 4;  // whole 16-byte chunks; an odd m leaves one 8-byte tail
     const uint4* src = reinterpret_cast<const uint4*>(sorted + b0);
@@ -1111,7 +1128,9 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
                                                             float dx, float dt, const uint32_t* __restrict__ occ,
                                                             const uint4* __restrict__ occr,
                                                             const int32_t* __restrict__ cs, UptakeSet u,
-                                                            int32_t* __restrict__ list) {
+                                                            int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     __shared__ uint32_t rec[STAGED ? kBlock * kRecDw : 1];
     const int64_t b0 = (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * kBlock;
     const int64_t i = b0 + threadIdx.x;
@@ -1136,8 +1155,8 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
         return;
     }
     // the tile's records, coalesced, into LDS (after the walk, so that the
-    // walk's registers are free again)
-    const int m = (int)((int64_t)n - b0 < kBlock ? (int64_t)n - b0 : kBlock);
+    // walk's registers are free again); none past a device-resident count
+    const int m = (int)max<int64_t>(0, min<int64_t>((int64_t)n - b0, kBlock));
     const int ndw = m * kRecDw;
     const int nq = ndw / 4;  // whole 16-byte chunks; an odd m leaves one 8-byte tail
     const uint4* src = reinterpret_cast<const uint4*>(sorted + b0);
@@ -1337,23 +1356,25 @@ hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out) {
 }
 
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
-    const int32_t m = n > ctx->se_list_n ? n : ctx->se_list_n;
+    const int32_t m = ctx->devn ? ctx->P.capacity : (n > ctx->se_list_n ? n : ctx->se_list_n);
     if (m == 0) return hipSuccess;
     const bool ranked = ctx->cl_mode == 1;  // what the previous gather wrote, cleared here
     hipLaunchKernelGGL(fm_keys, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells, n, ctx->N,
                        ctx->P.dx, ctx->keys[0], ctx->se_list, ctx->se_list_n, ctx->se, ctx->P.capacity + 1,
-                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked);
+                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked, dev_count(ctx),
+                       dev_count(ctx, DN_LIST));
     ctx->se_list_n = 0;
     return hipGetLastError();
 }
 
 hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
-    if (n == 0) return hipSuccess;
+    const int32_t gn = grid_count(ctx, n);
+    if (gn == 0) return hipSuccess;
     const int key_bits = 3 * ceil_log2(ctx->N) + 1;
-    if (n <= kSmallMax) {
+    if (gn <= kSmallMax) {
         hipLaunchKernelGGL(radix_sort_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ctx->keys[0], ctx->vals[0],
                            ctx->keys[1], ctx->vals[1], ctx->out_order, ctx->out_keys, n,
-                           plan_sort(key_bits, kSmallBits));
+                           plan_sort(key_bits, kSmallBits), dev_count(ctx));
         return hipGetLastError();
     }
     const SortPlan plan = plan_sort(key_bits, 9);
@@ -1371,16 +1392,18 @@ hipError_t fm_sort_launch(gell_ctx* ctx, int32_t n) {
 }
 
 hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
-    if (n == 0) return hipSuccess;
+    const int32_t gn = grid_count(ctx, n);
+    if (gn == 0) return hipSuccess;
     const bool blocks = ctx->tune[GELL_KNOB_FM_PREFETCH] == 5;
     if (blocks)
         hipLaunchKernelGGL(blk_reset, dim3((unsigned)((ctx->nmblk + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, ctx->mblk, ctx->nmblk, ctx->n_out);
     const bool ranked = ctx->tune[GELL_KNOB_FM_PREFETCH] == 9;
-    hipLaunchKernelGGL(fm_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
+    hipLaunchKernelGGL(fm_gather, dim3((gn + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
                        ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ranked ? nullptr : ctx->se,
                        ctx->se_list, ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out,
-                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked);
+                       ranked ? reinterpret_cast<uint32_t*>(ctx->occr) : ctx->occ, (int)ranked, dev_count(ctx),
+                       dev_count_out(ctx, DN_LIST));
     ctx->se_list_n = n;
     ctx->cl_mode = ranked ? 1 : 0;
     if (ranked) {
@@ -1388,11 +1411,12 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
         const unsigned rb = (unsigned)ctx->rank_blocks;
         hipLaunchKernelGGL(rank_count, dim3(rb), dim3(kBlock), 0, ctx->stream, ctx->occr, ctx->occr_words,
                            ctx->rank_tot);
-        hipLaunchKernelGGL(rank_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->rank_tot, (int32_t)rb, ctx->cs, n);
+        hipLaunchKernelGGL(rank_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->rank_tot, (int32_t)rb, ctx->cs, n,
+                           dev_count(ctx));
         hipLaunchKernelGGL(rank_write, dim3(rb), dim3(kBlock), 0, ctx->stream, ctx->occr, ctx->occr_words,
                            ctx->rank_tot, ctx->cs);
-        hipLaunchKernelGGL(fm_cs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->out_keys,
-                           ctx->nb, n, ctx->occr, ctx->cs);
+        hipLaunchKernelGGL(fm_cs, dim3((gn + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->out_keys,
+                           ctx->nb, n, ctx->occr, ctx->cs, dev_count(ctx));
     }
     return hipGetLastError();
 }
@@ -1403,8 +1427,10 @@ bool fm_fuses_uptake(const gell_ctx* ctx) {
 }
 
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* uptake_list) {
-    if (n == 0) return hipSuccess;
-    const dim3 grid((n + kBlock - 1) / kBlock);
+    const int32_t gn = grid_count(ctx, n);
+    if (gn == 0) return hipSuccess;
+    const dim3 grid((gn + kBlock - 1) / kBlock);
+    const int32_t* np = dev_count(ctx);
     int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
     UptakeSet ua{};
     if (uptake_list) {
@@ -1425,36 +1451,36 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int3
         const gell_cell* so = ctx->sorted;
         if (v == 7)
             hipLaunchKernelGGL((fm_force_move_xcd<true, false>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list, np);
         else if (v == 8)
             hipLaunchKernelGGL((fm_force_move_xcd<false, false>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list, np);
         else if (aligned16)
             hipLaunchKernelGGL((fm_force_move_xcd<true, true>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list, np);
         else
             hipLaunchKernelGGL((fm_force_move_xcd<false, true>), grid, dim3(kBlock), 0, ctx->stream, so, ctx->nb,
-                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list);
+                               ctx->se, cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, ctx->occr, ctx->cs, ua, uptake_list, np);
         return hipGetLastError();
     }
     if (v == 4)
         hipLaunchKernelGGL(fm_force_move_staged, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, np);
     else if (v == 6)
         hipLaunchKernelGGL(fm_force_move<6>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, np);
     else if (v == 3)
         hipLaunchKernelGGL(fm_force_move<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, np);
     else if (v == 2)
         hipLaunchKernelGGL(fm_force_move<2>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, np);
     else if (v == 1)
         hipLaunchKernelGGL(fm_force_move<1>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, np);
     else
         hipLaunchKernelGGL(fm_force_move<0>, grid, dim3(kBlock), 0, ctx->stream, ctx->sorted, ctx->nb, ctx->se,
-                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ);
+                           cells, n, ctx->N, ctx->P.dx, ctx->P.dt_mech, ctx->occ, np);
     return hipGetLastError();
 }
 

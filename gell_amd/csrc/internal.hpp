@@ -55,6 +55,17 @@ struct gell_ctx {
     int64_t rank_blocks = 0;
     int cl_mode = 0;              // what the last gather maintained: 0 se + occ, 1 occr + cs
 
+    // Device-resident counts (set while a graph-captured biology step is
+    // built, gell_ensemble_bio_step): with devn the kernels read the cell
+    // count and the list counts below from dn instead of launch arguments,
+    // and the launch grids cover the capacity, so that a captured biology
+    // step stays valid whatever birth and death do to the count.
+    // dn[DN_CELLS] cells, dn[DN_LIST] entries of the last gather's voxel
+    // list, dn[DN_MAP + s] entries of substrate s's uptake-map list,
+    // dn[DN_STEP..+1] the biology step (uint64) birth draws with.
+    int32_t* dn = nullptr;
+    bool devn = false;
+
     // Diffusion, per substrate.
     float* field[GELL_MAX_SUBSTRATES] = {};
     float* rate[GELL_MAX_SUBSTRATES] = {};
@@ -133,6 +144,13 @@ enum Phase { PH_FM_KEYS = 0, PH_FM_SORT, PH_FM_GATHER, PH_FM_FORCE, PH_PRELOD, P
 // Records the message of a failed context-free call (read back by
 // gell_last_error(NULL)) and returns `code`.
 int fail_global(int code, const std::string& msg);
+
+enum DevCount { DN_CELLS = 0, DN_LIST = 1, DN_MAP = 2, DN_STEP = 2 + GELL_MAX_SUBSTRATES, DN_SIZE = DN_STEP + 2 };
+// Count pointer for a kernel (null unless the counts are device-resident).
+inline const int32_t* dev_count(const gell_ctx* c, int k = DN_CELLS) { return c->devn ? c->dn + k : nullptr; }
+inline int32_t* dev_count_out(const gell_ctx* c, int k) { return c->devn ? c->dn + k : nullptr; }
+// Cells a launch's grid must cover.
+inline int32_t grid_count(const gell_ctx* c, int32_t n) { return c->devn ? c->P.capacity : n; }
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);

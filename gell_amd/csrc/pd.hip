@@ -86,15 +86,28 @@ __device__ __forceinline__ Bio bio_of(const CellRegs& c) {
 
 // Birthkernel's per-cell part (PDkernel.cuh:28-49): draw, O2 lookup,
 // Volume_update, Phase_update_ki67adv.
+// With a device-resident count (np) the step comes from stepp and a full
+// array (n >= capacity) updates nothing, as the host-count path skips the
+// launch then (PDkernel.cuh:31).
+__device__ __forceinline__ void step_of(const uint32_t* stepp, uint32_t& s0, uint32_t& s1) {
+    if (stepp) {
+        s0 = stepp[0];
+        s1 = stepp[1];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void birth_update(gell_cell* __restrict__ cells, int32_t n,
                                                        const float* __restrict__ p0, int32_t N, float dx,
                                                        float dt_bio, uint32_t k0, uint32_t k1, uint32_t s0,
                                                        uint32_t s1, int32_t* __restrict__ flags,
-                                                       int32_t* __restrict__ blk) {
+                                                       int32_t* __restrict__ blk, const int32_t* __restrict__ np,
+                                                       const uint32_t* __restrict__ stepp, int32_t capacity) {
     __shared__ int lds4[kBlock / 64];
+    n = count_of(n, np);
+    step_of(stepp, s0, s1);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool divide = false;
-    if (i < n) {
+    if (i < n && n < capacity) {
         CellRegs c = load_cell(cells, i);
         if (csign(c)) {
             const uint4 r = philox4x32_10(make_uint4((uint32_t)i, s0, s1, 0u), k0, k1);
@@ -118,8 +131,8 @@ __global__ __launch_bounds__(kBlock) void birth_update(gell_cell* __restrict__ c
             cset_sign(c, bio.sign);
             store_cell(cells, i, c);
         }
-        flags[i] = divide ? 1 : 0;
     }
+    if (i < n) flags[i] = divide ? 1 : 0;
     int total;
     block_rank(divide, lds4, total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
@@ -129,8 +142,12 @@ __global__ __launch_bounds__(kBlock) void birth_update(gell_cell* __restrict__ c
 __global__ __launch_bounds__(kBlock) void birth_divide(gell_cell* __restrict__ cells, int32_t n, int32_t capacity,
                                                        const int32_t* __restrict__ flags,
                                                        const int32_t* __restrict__ blk_off, uint32_t k0, uint32_t k1,
-                                                       uint32_t s0, uint32_t s1, int32_t* __restrict__ slots) {
+                                                       uint32_t s0, uint32_t s1, int32_t* __restrict__ slots,
+                                                       const int32_t* __restrict__ np,
+                                                       const uint32_t* __restrict__ stepp) {
     __shared__ int lds4[kBlock / 64];
+    n = count_of(n, np);
+    step_of(stepp, s0, s1);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool flag = i < n && flags[i] != 0;
     int total;
@@ -183,8 +200,9 @@ __global__ __launch_bounds__(kBlock) void birth_divide(gell_cell* __restrict__ c
 }
 
 __global__ __launch_bounds__(kBlock) void death_count(const gell_cell* __restrict__ cells, int32_t n,
-                                                      int32_t* __restrict__ blk) {
+                                                      int32_t* __restrict__ blk, const int32_t* __restrict__ np) {
     __shared__ int lds4[kBlock / 64];
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n && (reinterpret_cast<const uint32_t*>(cells + i)[W_SIGN] & 0xFFu) != 0;
     int total;
@@ -194,8 +212,9 @@ __global__ __launch_bounds__(kBlock) void death_count(const gell_cell* __restric
 
 __global__ __launch_bounds__(kBlock) void death_scatter(const gell_cell* __restrict__ cells, int32_t n,
                                                         const int32_t* __restrict__ blk_off,
-                                                        gell_cell* __restrict__ out) {
+                                                        gell_cell* __restrict__ out, const int32_t* __restrict__ np) {
     __shared__ int lds4[kBlock / 64];
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n && (reinterpret_cast<const uint32_t*>(cells + i)[W_SIGN] & 0xFFu) != 0;
     int total;
@@ -213,23 +232,46 @@ __global__ __launch_bounds__(kBlock) void copy_counted(const uint2* __restrict__
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) dst[i] = src[i];
 }
 
+// The new device-resident count after birth: n plus the births that fit
+// below capacity - 1 (PDkernel.cuh:51-54); unchanged for a full array.
+__global__ void birth_count(int32_t* __restrict__ np, const int32_t* __restrict__ births, int32_t capacity) {
+    const int32_t n = *np;
+    if (n <= 0 || n >= capacity) return;
+    const int64_t room = (int64_t)capacity - 1 - n;
+    const int64_t b = *births < room ? *births : (room > 0 ? room : 0);
+    *np = (int32_t)(n + b);
+}
+
+// The new device-resident count after death: the survivors.
+__global__ void death_set_count(int32_t* __restrict__ np, const int32_t* __restrict__ live) { *np = *live; }
+
 }  // namespace
 
 hipError_t birth_begin(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t step) {
     hipStream_t st = ctx->stream;
-    if (n <= 0 || n >= ctx->P.capacity) {
+    const bool dev = ctx->devn;
+    if (!dev && (n <= 0 || n >= ctx->P.capacity)) {
         // PDkernel.cuh:31: nothing happens once the array is full.
         if (n > 0) return hipMemsetAsync(ctx->slots, 0xFF, sizeof(int32_t) * (size_t)n, st);
         return hipSuccess;
     }
-    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    // with device-resident counts every case runs on the device, sized for
+    // the capacity, and the step is read from dn[DN_STEP]
+    const int32_t gn = grid_count(ctx, n);
+    const unsigned blocks = (unsigned)((gn + kBlock - 1) / kBlock);
     const uint32_t k0 = (uint32_t)ctx->P.seed, k1 = (uint32_t)(ctx->P.seed >> 32);
     const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
+    const int32_t* np = dev_count(ctx);
+    const uint32_t* stepp = dev ? reinterpret_cast<const uint32_t*>(ctx->dn + DN_STEP) : nullptr;
     hipLaunchKernelGGL(birth_update, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->field[0], ctx->N, ctx->P.dx,
-                       ctx->P.dt_bio, k0, k1, s0, s1, ctx->flags, ctx->blk);
+                       ctx->P.dt_bio, k0, k1, s0, s1, ctx->flags, ctx->blk, np, stepp, ctx->P.capacity);
     hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanThreads), 0, st, ctx->blk, (int32_t)blocks, ctx->d_total);
     hipLaunchKernelGGL(birth_divide, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->P.capacity, ctx->flags,
-                       ctx->blk, k0, k1, s0, s1, ctx->slots);
+                       ctx->blk, k0, k1, s0, s1, ctx->slots, np, stepp);
+    if (dev) {
+        hipLaunchKernelGGL(birth_count, dim3(1), dim3(1), 0, st, ctx->dn + DN_CELLS, ctx->d_total, ctx->P.capacity);
+        return hipGetLastError();
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
@@ -252,16 +294,23 @@ hipError_t birth_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, uint64_t ste
 }
 
 hipError_t death_begin(gell_ctx* ctx, gell_cell* cells, int32_t n) {
-    if (n <= 0) return hipSuccess;
+    const bool dev = ctx->devn;
+    if (!dev && n <= 0) return hipSuccess;
     hipStream_t st = ctx->stream;
-    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk);
+    const int32_t gn = grid_count(ctx, n);
+    const int32_t* np = dev_count(ctx);
+    const unsigned blocks = (unsigned)((gn + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk, np);
     hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanThreads), 0, st, ctx->blk, (int32_t)blocks, ctx->d_total);
-    hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk, ctx->compact);
-    const int64_t words = (int64_t)n * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
-    const int64_t cblocks = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->blk, ctx->compact, np);
+    const int64_t words = (int64_t)gn * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
+    const int64_t cblocks = std::max<int64_t>(1, std::min<int64_t>((words + kBlock - 1) / kBlock, 8192));
     hipLaunchKernelGGL(copy_counted, dim3((unsigned)cblocks), dim3(kBlock), 0, st,
                        reinterpret_cast<const uint2*>(ctx->compact), reinterpret_cast<uint2*>(cells), ctx->d_total);
+    if (dev) {
+        hipLaunchKernelGGL(death_set_count, dim3(1), dim3(1), 0, st, ctx->dn + DN_CELLS, ctx->d_total);
+        return hipGetLastError();
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(ctx->h_total, ctx->d_total, sizeof(int32_t), hipMemcpyDeviceToHost, st);
@@ -351,9 +400,9 @@ extern "C" int gell_death_noalloc(gell_cell* d_cells, int32_t n, void* stream, i
     // every word the kernels read is written earlier in the same call: the
     // counts by death_count, offsets and total by scan_small, the records by
     // death_scatter; nothing has to start at zero
-    hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk);
+    hipLaunchKernelGGL(death_count, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk, nullptr);
     hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanThreads), 0, st, blk, (int32_t)blocks, total);
-    hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk, compact);
+    hipLaunchKernelGGL(death_scatter, dim3(blocks), dim3(kBlock), 0, st, d_cells, n, blk, compact, nullptr);
     const int64_t words = (int64_t)n * (int64_t)(sizeof(gell_cell) / sizeof(uint2));
     const int64_t cblocks = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
     hipLaunchKernelGGL(copy_counted, dim3((unsigned)cblocks), dim3(kBlock), 0, st,

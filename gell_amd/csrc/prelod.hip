@@ -30,7 +30,9 @@ constexpr int kBlock = 256;
 
 // Rates of the voxels on a list of Morton codes back to zero.
 __global__ __launch_bounds__(kBlock) void prelod_clear(const int32_t* __restrict__ list, int32_t m,
-                                                       float* __restrict__ rate, int32_t N) {
+                                                       float* __restrict__ rate, int32_t N,
+                                                       const int32_t* __restrict__ mp) {
+    m = count_of(m, mp);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int32_t c = list[i];
@@ -57,7 +59,9 @@ __global__ __launch_bounds__(kBlock) void prelod_clear_all(float* __restrict__ r
 
 __global__ __launch_bounds__(kBlock) void prelod_accumulate(const gell_cell* __restrict__ cells, int32_t n,
                                                             int32_t N, float dx, UptakeSet u,
-                                                            int32_t* __restrict__ list) {
+                                                            int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ np) {
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t V = (int64_t)N * N * N;
     bool take = false;
@@ -79,8 +83,11 @@ __global__ __launch_bounds__(kBlock) void prelod_accumulate(const gell_cell* __r
 // list (a voxel listed several times is written with the same value).
 __global__ __launch_bounds__(kBlock) void prelod_finalize(const int32_t* __restrict__ list, int32_t n,
                                                           const long long* __restrict__ acc, float* __restrict__ rate,
-                                                          int32_t N) {
+                                                          int32_t N, const int32_t* __restrict__ np,
+                                                          int32_t* __restrict__ map_np) {
+    n = count_of(n, np);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (map_np && i == 0) *map_np = n;  // the next clear of this map reads that many list entries
     if (i >= n) return;
     const int32_t c = list[i];
     if (c >= 0) rate[lin_of_code((uint32_t)c, N)] = from_fixed(acc[c]);
@@ -88,7 +95,9 @@ __global__ __launch_bounds__(kBlock) void prelod_finalize(const int32_t* __restr
 
 // Accumulators of the voxels on the list back to zero, every substrate of
 // the set (after the finalize has read them).
-__global__ __launch_bounds__(kBlock) void prelod_zero_acc(const int32_t* __restrict__ list, int32_t m, UptakeSet u) {
+__global__ __launch_bounds__(kBlock) void prelod_zero_acc(const int32_t* __restrict__ list, int32_t m, UptakeSet u,
+                                                          const int32_t* __restrict__ mp) {
+    m = count_of(m, mp);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int32_t c = list[i];
@@ -139,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void xspan_scan(const float* __restrict__ r
 // list that built it.
 void clear_map(gell_ctx* ctx, int s) {
     hipStream_t st = ctx->stream;
-    const int32_t m = ctx->rate_list_n[s];
+    // with device-resident counts the list clear runs over the device count
+    const int32_t m = ctx->devn ? ctx->P.capacity : ctx->rate_list_n[s];
     const int64_t lines = (int64_t)ctx->N * ctx->N;
     if (ctx->rate_handed_out[s])
         hipLaunchKernelGGL(prelod_clear_all, dim3((unsigned)((ctx->V + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
@@ -149,7 +159,7 @@ void clear_map(gell_ctx* ctx, int s) {
                            dim3(kBlock), 0, st, ctx->xspan[s], ctx->rate[s], ctx->N);
     else if (m > 0)
         hipLaunchKernelGGL(prelod_clear, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ctx->rate_list_ptr[s],
-                           m, ctx->rate[s], ctx->N);
+                           ctx->rate_list_n[s], ctx->rate[s], ctx->N, dev_count(ctx, DN_MAP + s));
     ctx->rate_list_n[s] = 0;
     ctx->rate_handed_out[s] = false;
 }
@@ -171,14 +181,16 @@ hipError_t prelod_launch(gell_ctx* ctx, int s, const gell_cell* cells, int32_t n
     hipStream_t st = ctx->stream;
     clear_map(ctx, s);
     ctx->rate_list_ptr[s] = ctx->rate_list[s];
-    if (n > 0) {
-        const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    const int32_t gn = grid_count(ctx, n);
+    if (gn > 0) {
+        const unsigned blocks = (unsigned)((gn + kBlock - 1) / kBlock);
         const UptakeSet u = uptake_set(ctx, s, 1);
+        const int32_t* np = dev_count(ctx);
         hipLaunchKernelGGL(prelod_accumulate, dim3(blocks), dim3(kBlock), 0, st, cells, n, ctx->N, ctx->P.dx, u,
-                           ctx->rate_list[s]);
+                           ctx->rate_list[s], np);
         hipLaunchKernelGGL(prelod_finalize, dim3(blocks), dim3(kBlock), 0, st, ctx->rate_list[s], n, ctx->acc[s],
-                           ctx->rate[s], ctx->N);
-        hipLaunchKernelGGL(prelod_zero_acc, dim3(blocks), dim3(kBlock), 0, st, ctx->rate_list[s], n, u);
+                           ctx->rate[s], ctx->N, np, dev_count_out(ctx, DN_MAP + s));
+        hipLaunchKernelGGL(prelod_zero_acc, dim3(blocks), dim3(kBlock), 0, st, ctx->rate_list[s], n, u, np);
         ctx->rate_list_n[s] = n;
     }
     ctx->xspan_valid[s] = false;  // rebuilt by the first substep that reads the spans
@@ -197,9 +209,10 @@ int32_t* prelod_fused_list(gell_ctx* ctx) {
 
 hipError_t prelod_fused_finalize(gell_ctx* ctx, int s, const int32_t* list, int32_t n) {
     clear_map(ctx, s);
-    if (n > 0)
-        hipLaunchKernelGGL(prelod_finalize, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, list, n,
-                           ctx->acc[s], ctx->rate[s], ctx->N);
+    const int32_t gn = grid_count(ctx, n);
+    if (gn > 0)
+        hipLaunchKernelGGL(prelod_finalize, dim3((gn + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, list, n,
+                           ctx->acc[s], ctx->rate[s], ctx->N, dev_count(ctx), dev_count_out(ctx, DN_MAP + s));
     ctx->rate_list_ptr[s] = list;
     ctx->rate_list_n[s] = n;
     ctx->xspan_valid[s] = false;
@@ -207,9 +220,10 @@ hipError_t prelod_fused_finalize(gell_ctx* ctx, int s, const int32_t* list, int3
 }
 
 hipError_t prelod_fused_zero(gell_ctx* ctx, const int32_t* list, int32_t n) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(prelod_zero_acc, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, list, n,
-                       prelod_uptake_set(ctx));
+    const int32_t gn = grid_count(ctx, n);
+    if (gn <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prelod_zero_acc, dim3((gn + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, list, n,
+                       prelod_uptake_set(ctx), dev_count(ctx));
     return hipGetLastError();
 }
 
