@@ -333,6 +333,10 @@ def ensemble_main(args) -> int:
     diff = rt.float_loop_count(0.1, 0.01)
     ens = rt.Ensemble([P for P, _, _ in reps])
     ens.autotune()
+    knobs = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.knob)}
+    for m in ens.members:  # --knob overrides the autotuner's choice
+        for k, v in knobs.items():
+            m.tune(k, v)
     bufs = []
     for P, cells, _ in reps:
         b = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
@@ -368,6 +372,8 @@ def ensemble_main(args) -> int:
     for (P, cells, n), b in zip(reps, bufs):
         ctx = rt.Context(P)
         ctx.autotune()
+        for k, v in knobs.items():
+            ctx.tune(k, v)
         b.upload(cells)
         ctx.sync()
         t0 = time.perf_counter()

@@ -1355,6 +1355,15 @@ hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out) {
     return hipGetLastError();
 }
 
+// The variant a launch for n cells runs: the rank-compacted list (9) costs
+// four small extra launches per step, which below kRankMin cells outweigh
+// what its walk saves, so there variant 7 runs instead (same bits).
+constexpr int32_t kRankMin = 131072;
+int fm_variant(const gell_ctx* ctx, int32_t n) {
+    const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    return (v == 9 && grid_count(ctx, n) < kRankMin) ? 7 : v;
+}
+
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     const int32_t m = ctx->devn ? ctx->P.capacity : (n > ctx->se_list_n ? n : ctx->se_list_n);
     if (m == 0) return hipSuccess;
@@ -1398,7 +1407,7 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
     if (blocks)
         hipLaunchKernelGGL(blk_reset, dim3((unsigned)((ctx->nmblk + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, ctx->mblk, ctx->nmblk, ctx->n_out);
-    const bool ranked = ctx->tune[GELL_KNOB_FM_PREFETCH] == 9;
+    const bool ranked = fm_variant(ctx, n) == 9;
     hipLaunchKernelGGL(fm_gather, dim3((gn + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, cells,
                        ctx->out_order, n, ctx->N, ctx->P.dx, ctx->sorted, ctx->nb, ranked ? nullptr : ctx->se,
                        ctx->se_list, ctx->out_keys, blocks ? ctx->mblk : nullptr, ctx->n_out,
@@ -1431,7 +1440,7 @@ hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int3
     if (gn == 0) return hipSuccess;
     const dim3 grid((gn + kBlock - 1) / kBlock);
     const int32_t* np = dev_count(ctx);
-    int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
+    int v = fm_variant(ctx, n);
     UptakeSet ua{};
     if (uptake_list) {
         // only the XCD-ordered kernels accumulate (fm_fuses_uptake)

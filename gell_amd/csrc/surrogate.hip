@@ -79,6 +79,14 @@ struct gell_surrogate {
     float* red_part = nullptr;
     int adam_t = 0;
     int accumulate = 0;  // add this call's gradients to the stored ones
+    // The in-loop gate (gell_surrogate_advance): the largest measured ratio
+    // of a prediction's true error to its one-substep residual (< 0: none
+    // measured yet), the calls since the last exact check, the error spent in
+    // the current budget window, and counts of each kind of call.
+    double ratio = -1.0;
+    int since_check = 0;
+    double spent = 0.0;
+    int64_t n_checked = 0, n_accepted = 0, n_fallback = 0;
     std::vector<void*> allocs;
 };
 
@@ -264,6 +272,7 @@ void gell_surrogate_default_config(gell_surrogate_config* c) {
     c->boundary = 38.f;
     c->rate_scale = 0.1f;
     c->out_scale = 1.f;
+    c->check_every = 6;
     c->seed = 7;
 }
 
@@ -451,6 +460,39 @@ int gell_surrogate_adam(gell_surrogate* s, float lr) {
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("adam: ") + hipGetErrorString(e));
 }
 
+namespace {
+
+// ||a - b|| / ||a|| over n floats (fixed-order reduction); synchronises.
+hipError_t rel_change(gell_surrogate* s, const float* a, const float* b, int64_t n, double* out) {
+    hipLaunchKernelGGL(diff_norms, dim3(kRedBlocks), dim3(256), 0, s->stream, a, b, n, s->red_part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->h_red, s->red_part, sizeof(float) * 2 * kRedBlocks, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return e;
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < kRedBlocks; ++b) {
+        num += s->h_red[2 * b];
+        den += s->h_red[2 * b + 1];
+    }
+    *out = den > 0.0 ? std::sqrt(num / den) : 0.0;
+    return hipSuccess;
+}
+
+}  // namespace
+
+// The gate. Every call predicts and measures the one-substep residual r of
+// the prediction (its relative change under one real substep). On the first
+// call and every check_every-th call after, the K real substeps run as well
+// and the field keeps their exact result; the prediction's true relative
+// error e against them calibrates ratio = max(e / r) over the checks (times
+// a safety factor). Between checks a prediction is kept only when the error
+// budget allows it: its estimated error ratio * r, added to the estimates of
+// the predictions already kept in the current window
+// (gell_surrogate_reset_budget starts a new one), stays within tol; otherwise
+// the K real substeps run. The error a window adds to the field is thus
+// bounded by tol, as far as the calibrated estimate holds; with tol = 0
+// every step is exact.
 int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float tol, int32_t* used, float* residual) {
     if (!s || !ctx || !used) return sfail(s, GELL_E_ARG, "null argument");
     if (ctx->N != s->N) return sfail(s, GELL_E_ARG, "grid size differs from the surrogate's");
@@ -459,6 +501,7 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     hipStream_t st = s->stream;
     const int64_t V = ctx->V;
     float* const field = ctx->field[sub];
+    *used = 0;
     hipError_t e = gell::join_substrates(ctx);
     if (e == hipSuccess) e = forward(s, field, ctx->rate[sub]);
     if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, s->pred, st);
@@ -469,25 +512,52 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
         e = gell::lod_substep_launch(ctx, sub, st);
         ctx->field[sub] = field;
     }
-    if (e == hipSuccess) hipLaunchKernelGGL(diff_norms, dim3(kRedBlocks), dim3(256), 0, st, s->check, s->pred, V, s->red_part);
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(s->h_red, s->red_part, sizeof(float) * 2 * kRedBlocks, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    double res = 0.0;
+    if (e == hipSuccess) e = rel_change(s, s->check, s->pred, V, &res);
     if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
-    double num = 0.0, den = 0.0;
-    for (int b = 0; b < kRedBlocks; ++b) {
-        num += s->h_red[2 * b];
-        den += s->h_red[2 * b + 1];
+    if (residual) *residual = (float)res;
+    const int every = s->cfg.check_every > 0 ? s->cfg.check_every : 1;
+    if (s->ratio < 0.0 || s->since_check + 1 >= every) {
+        // exact check: the K real substeps, kept; the prediction's error calibrates the estimate
+        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
+        double err = 0.0;
+        if (e == hipSuccess) e = rel_change(s, field, s->pred, V, &err);
+        if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance check: ") + hipGetErrorString(e));
+        const double r = err / (res > 1e-30 ? res : 1e-30);
+        s->ratio = r > s->ratio ? r : s->ratio;
+        s->since_check = 0;
+        ++s->n_checked;
+        return GELL_OK;
     }
-    const float res = den > 0.0 ? (float)std::sqrt(num / den) : 0.f;
-    if (residual) *residual = res;
-    *used = res <= tol ? 1 : 0;
-    if (*used) {
+    ++s->since_check;
+    const double est = 1.5 * s->ratio * res;
+    if (s->spent + est <= (double)tol) {
+        s->spent += est;
+        *used = 1;
+        ++s->n_accepted;
         e = hipMemcpyAsync(field, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
     } else {
+        ++s->n_fallback;
         for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
     }
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
+}
+
+int gell_surrogate_reset_budget(gell_surrogate* s) {
+    if (!s) return sfail(nullptr, GELL_E_ARG, "null surrogate");
+    s->spent = 0.0;
+    return GELL_OK;
+}
+
+int gell_surrogate_gate_stats(const gell_surrogate* s, int64_t* checked, int64_t* accepted, int64_t* fallback,
+                              double* ratio, double* spent) {
+    if (!s) return sfail(nullptr, GELL_E_ARG, "null surrogate");
+    if (checked) *checked = s->n_checked;
+    if (accepted) *accepted = s->n_accepted;
+    if (fallback) *fallback = s->n_fallback;
+    if (ratio) *ratio = s->ratio;
+    if (spent) *spent = s->spent;
+    return GELL_OK;
 }
 
 // Single 3x3x3 layer on an S^3 grid, for checking the kernels one at a time.

@@ -76,7 +76,7 @@ class SurrogateConfig(C.Structure):
         ("boundary", C.c_float),
         ("rate_scale", C.c_float),
         ("out_scale", C.c_float),
-        ("reserved_", C.c_int32),
+        ("check_every", C.c_int32),
         ("seed", C.c_uint64),
     ]
 
@@ -159,6 +159,8 @@ def load_library() -> C.CDLL:
         "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, i32, P(C.c_float), vp]),
         "gell_surrogate_adam": (C.c_int, [vp, C.c_float]),
         "gell_surrogate_advance": (C.c_int, [vp, vp, i32, C.c_float, P(i32), P(C.c_float)]),
+        "gell_surrogate_reset_budget": (C.c_int, [vp]),
+        "gell_surrogate_gate_stats": (C.c_int, [vp, P(i64), P(i64), P(i64), P(C.c_double), P(C.c_double)]),
         "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
         "gell_nn_conv3_wgrad": (C.c_int, [vp, vp, vp, i32, i32, i32]),
         "gell_io_last_error": (C.c_char_p, []),
@@ -550,6 +552,16 @@ class Surrogate:
         used, res = C.c_int32(), C.c_float()
         self._check(self.lib.gell_surrogate_advance(self.h, ctx.h, s, tol, C.byref(used), C.byref(res)), "advance")
         return bool(used.value), res.value
+
+    def reset_budget(self) -> None:
+        self._check(self.lib.gell_surrogate_reset_budget(self.h), "reset_budget")
+
+    def gate_stats(self) -> dict:
+        c, a, f = C.c_int64(), C.c_int64(), C.c_int64()
+        r, sp = C.c_double(), C.c_double()
+        self._check(self.lib.gell_surrogate_gate_stats(self.h, C.byref(c), C.byref(a), C.byref(f), C.byref(r),
+                                                       C.byref(sp)), "gate_stats")
+        return {"checked": c.value, "accepted": a.value, "fallback": f.value, "ratio": r.value, "spent": sp.value}
 
     def close(self) -> None:
         if getattr(self, "h", None):

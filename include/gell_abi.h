@@ -233,7 +233,10 @@ int gell_sync(gell_ctx* ctx);
  *                              bits and the rank of its first occupied voxel
  *                              (16 bytes), and per occupied voxel its first
  *                              live slot (4 bytes), instead of the 8-byte
- *                              Morton-indexed range of every voxel.
+ *                              Morton-indexed range of every voxel; for
+ *                              fewer than 131072 cells (capacity, with
+ *                              device-resident counts) 7 runs instead, the
+ *                              list's extra launches costing more there.
  *                              Variants 7-9 also accumulate the uptake maps
  *                              in gell_mech_step.
  *                              Default 9.
@@ -415,9 +418,10 @@ int gell_ensemble_stats(const gell_ensemble* ens, int64_t* captures, int64_t* la
  * substep, Main.cu:121-123): a four-level 3-D U-Net (16..128 channels, BF16
  * matrix-core convolutions with FP32 accumulation) on the (N/2)^3 grid that
  * maps (field, uptake map) to the field after K substeps with that map held
- * fixed. Forward, gradients and Adam run on the device; the in-loop call
- * checks each prediction with one real substep and falls back to K substeps
- * when the relative change exceeds the tolerance.
+ * fixed. Forward, gradients and Adam run on the device. In the loop
+ * (gell_surrogate_advance) plain LOD stays the fallback and the judge: an
+ * error budget decides whether a prediction stands in for the K substeps,
+ * calibrated against the real K substeps every few calls.
  * ------------------------------------------------------------------------- */
 typedef struct gell_surrogate gell_surrogate;
 typedef struct {
@@ -426,7 +430,8 @@ typedef struct {
     float boundary;   /* boundary value (input scaling; outer layers) */
     float rate_scale; /* input scaling of the uptake map */
     float out_scale;  /* field units of one head unit */
-    int32_t reserved_;
+    int32_t check_every; /* gell_surrogate_advance: one call in this many runs
+                            the K real substeps to calibrate the gate (6) */
     uint64_t seed;    /* weight initialisation */
 } gell_surrogate_config;
 void gell_surrogate_default_config(gell_surrogate_config* cfg);
@@ -448,12 +453,25 @@ int gell_surrogate_gradients(gell_surrogate* sur, const float* field, const floa
                              int32_t accumulate, float* loss, float* grad_host);
 /* One Adam step (0.9, 0.999, 1e-8) on the last gradients. */
 int gell_surrogate_adam(gell_surrogate* sur, float lr);
-/* Advance substrate `s` of ctx by K substeps: the surrogate's prediction is
- * kept when one real substep changes it by at most `tol` (relative L2),
- * otherwise K LOD substeps run instead; *used tells which, *residual the
- * measured change. ctx must have grid N and run on the surrogate's stream.
- * Synchronises. */
+/* Advance substrate `s` of ctx by K substeps. Each call predicts and
+ * measures the prediction's residual r, its relative L2 change under one
+ * real substep (*residual). The first call and every check_every-th call
+ * after run the K real substeps too and keep their exact result; the
+ * prediction's true relative L2 error e against them calibrates the ratio
+ * e / r (the largest seen). Between those checks a prediction is kept
+ * (*used = 1) only while the estimated errors (1.5 ratio r) of the
+ * predictions kept in the current window add up to at most `tol`, relative
+ * L2; otherwise the K LOD substeps run (*used = 0). tol = 0 makes every
+ * call exact (bit for bit K substeps). ctx must have grid N and run on the
+ * surrogate's stream. Synchronises. */
 int gell_surrogate_advance(gell_surrogate* sur, gell_ctx* ctx, int32_t s, float tol, int32_t* used, float* residual);
+/* Start a new error-budget window (e.g. at each biology step). */
+int gell_surrogate_reset_budget(gell_surrogate* sur);
+/* Calls of gell_surrogate_advance so far by outcome (exact check, prediction
+ * kept, fallback to the K substeps), the calibrated error / residual ratio
+ * (< 0 before the first check) and the error spent in the window. */
+int gell_surrogate_gate_stats(const gell_surrogate* sur, int64_t* checked, int64_t* accepted, int64_t* fallback,
+                              double* ratio, double* spent);
 /* One of the network's 3x3x3 layers alone on an S^3 grid (S a multiple of
  * 8), for checking the kernels: x, y device BF16 tensors (channels-last);
  * w_master [cout][cin][27] and b[cout] on the host. With input_grad the call

@@ -147,13 +147,15 @@ def test_fm_carried_threshold_unequal_radii(oracle):
     st.close()
 
 
-@pytest.mark.parametrize("N,n,dense", [(13, 3000, True), (30, 3000, True), (64, 16385, False), (96, 20000, True)])
+@pytest.mark.parametrize("N,n,dense", [(30, 140000, True), (64, 140000, False), (96, 150000, True),
+                                       (128, 131072, False)])
 def test_fm_rank_list_matches_oracle(N, n, dense, oracle):
     """Force variant 9 keeps the rank-compacted cell list instead of the
-    Morton-indexed ranges; gell_voxel_ranges rebuilds the reference's s/e
-    from it. Keys, order, ranges and records against the oracle, with the
-    variant switched to 6 and back between steps (each clears what the other
-    left)."""
+    Morton-indexed ranges (from 131072 cells on); gell_voxel_ranges rebuilds
+    the reference's s/e from it. Keys, order, ranges and records against the
+    oracle, with the variant switched to 6 and back between steps (each
+    clears what the other left). N = 30: occupancy words clipped by the
+    grid's faces, ~5 cells per voxel."""
     P = params(N, capacity=n + 64)
     cells = mixed_cells(N, 10.0, n, P.capacity, seed=N + n + 1, dense=dense)
     st = GpuState(P, cells)

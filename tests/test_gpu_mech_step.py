@@ -42,7 +42,7 @@ def _run(P, cells, n, variant, plan, substeps=2):
     return cells_out, maps, fields
 
 
-@pytest.mark.parametrize("N,n,dense", [(64, 20000, False), (48, 15000, True), (128, 30001, False)])
+@pytest.mark.parametrize("N,n,dense", [(64, 20000, False), (48, 15000, True), (128, 140001, False)])
 @pytest.mark.parametrize("variant", [6, 7, 8, 9])
 def test_mech_step_equals_fm_update_and_prelod(N, n, dense, variant, oracle):
     log(f"[mech] N={N} n={n} dense={dense} variant={variant}")

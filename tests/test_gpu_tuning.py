@@ -28,7 +28,7 @@ def test_lod_variants_identical(N):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000), (13, 3000)])
+@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000), (13, 3000), (64, 140001)])
 def test_fm_variants_identical(N, n):
     """All force-kernel variants give the same bits; an odd count exercises
     the staged variant's 8-byte record tail, N = 100 Morton blocks that
