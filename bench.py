@@ -530,6 +530,13 @@ def surrogate_leg(args) -> dict:
     cfg = dict(CONFIGS["L"], n_sub=1)
     P, cells = make_workload(cfg, args.seed)
     n0 = cfg["cells"]
+    # a proliferating rim (clocks spread over their windows) and a necrotic
+    # core, so that the compared biology step has births and deaths to count
+    from gell_amd import spheroid
+
+    c = cfg["N"] * cfg["dx"] / 2.0
+    spheroid.radial_profile(cells, n0, (c, c, c), cfg["radius"], args.seed + 1)
+    rng = np.random.default_rng(args.seed + 2)
     ctx = rt.Context(P)
     ctx.autotune()
     buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
@@ -552,9 +559,13 @@ def surrogate_leg(args) -> dict:
     snap_cells = buf.download(rt.CELL_DTYPE, P.capacity)
     snap_field = ctx.field(0)
     snap_n = n
+    # 1 % of the cells dead at the start of the compared step (removed by its death)
+    snap_cells["sign"][:snap_n][rng.random(snap_n) < 0.01] = 0
 
     def bio_step(use_sur: bool, step: int):
         stats = dict(calls=0, used=0, residual=[], lod_substeps=0)
+        if use_sur:
+            sur.reset_budget()  # the error budget is per biology step
         m = ctx.birth(buf, snap_n, step)
         born = m - snap_n
         owed = 0
@@ -593,6 +604,7 @@ def surrogate_leg(args) -> dict:
         st["field"] = ctx.field(0)
         out[mode] = st
         log(f"[bench N] {mode}: {st['seconds'] * 1e3:.1f} ms, calls {st['calls']}, used {st['used']}")
+    sur_gate = sur.gate_stats()
     # the surrogate alone against the 50 substeps it replaces
     ctx.set_field(0, snap_field)
     pred = rt.DeviceBuffer(cfg["N"] ** 3 * 4)
@@ -635,6 +647,7 @@ def surrogate_leg(args) -> dict:
         "births": {"plain": pl["births"], "surrogate": su["births"]},
         "deaths": {"plain": pl["deaths"], "surrogate": su["deaths"]},
         "lod_substeps": {"plain": pl["lod_substeps"], "surrogate": su["lod_substeps"]},
+        "gate": sur_gate,
         "speedup_bio_step": pl["seconds"] / su["seconds"],
         "event_delta_frac": {
             k: (abs(su[k] - pl[k]) / pl[k] if pl[k] else float(su[k] != pl[k])) for k in ("births", "deaths")},

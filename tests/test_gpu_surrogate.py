@@ -1,0 +1,182 @@
+"""The LOD surrogate in the loop (gell_surrogate_advance) with the committed
+trained weights (gell_amd/weights/lod_surrogate_k50.npy): plain LOD stays the
+fallback and the judge.
+
+* tol = 0: every call is exact -- the field equals K = 50 real substeps under
+  the same uptake map, bit for bit (first call a calibration check, the rest
+  fall back);
+* an input far from anything the network was trained on (noise on the field)
+  is not kept: the call after the check falls back;
+* over one biology step with births and deaths (phase-profiled spheroid,
+  dead cells), the default tolerance 1e-3 keeps the field within 1e-3
+  relative L2 of the run whose every surrogate call is exact, and the birth
+  and death counts within 1 %;
+* ablation: on a relaxing field (the first mechanics step from a uniform
+  field) the trained network's prediction is closer to the 50 real substeps
+  than the method without it (keeping the field, i.e. zero weights, whose
+  prediction is the input itself)."""
+import numpy as np
+import pytest
+
+import bench
+from gell_amd import runtime as rt
+from gell_amd import spheroid
+from helpers import log, params, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+K = 50
+
+
+def weights():
+    return np.load(bench.SURROGATE_WEIGHTS, allow_pickle=False)
+
+
+def _state(n, seed, profile=True, mech=3):
+    P = params(N, capacity=2 * n, seed=seed)
+    R = 0.35 * N * 10.0
+    c = N * 5.0
+    cells = spheroid.make_cells(n, N, 10.0, R, seed, capacity=P.capacity)
+    if profile:
+        spheroid.radial_profile(cells, n, (c, c, c), R, seed + 1)
+    ctx = rt.Context(P)
+    buf = rt.DeviceBuffer(P.capacity * 72)
+    buf.upload(cells)
+    for _ in range(mech):
+        ctx.fm_update(buf, n)
+        ctx.prelod(0, buf, n)
+        for _ in range(11):
+            ctx.lod_substep(0)
+    ctx.sync()
+    return P, ctx, buf
+
+
+def _twin(P, ctx, buf):
+    """A second context in the same state (cells, field, uptake map)."""
+    t = rt.Context(P)
+    tb = rt.DeviceBuffer(P.capacity * 72)
+    tb.upload(buf.download(rt.CELL_DTYPE, P.capacity))
+    rt.upload_ptr(t.lib.gell_uptake_field(t.h, 0), ctx.uptake_field(0))
+    t.set_field(0, ctx.field(0))
+    return t, tb
+
+
+def test_tol0_gives_the_lod_substeps_bit_for_bit():
+    P, ctx, buf = _state(100_000, 11)
+    twin, tbuf = _twin(P, ctx, buf)
+    sur = rt.Surrogate(N, K)
+    sur.set_params(weights())
+    for call in range(3):
+        used, res = sur.advance(ctx, 0, 0.0)
+        assert not used
+        for _ in range(K):
+            twin.lod_substep(0)
+        assert np.array_equal(ctx.field(0).view(np.uint32), twin.field(0).view(np.uint32)), f"call {call}"
+    st = sur.gate_stats()
+    log(f"[surrogate] tol 0: {st}")
+    assert (st["checked"], st["accepted"], st["fallback"]) == (1, 0, 2)
+    for o in (sur, ctx, twin):
+        o.close()
+    buf.free()
+    tbuf.free()
+
+
+def test_input_off_the_training_data_falls_back():
+    P, ctx, buf = _state(100_000, 12)
+    twin, tbuf = _twin(P, ctx, buf)
+    sur = rt.Surrogate(N, K)
+    sur.set_params(weights())
+    rng = np.random.default_rng(3)
+    outcomes = []
+    for _ in range(2):
+        # noise on the field before each call (the substeps smooth it away)
+        noisy = ctx.field(0) + (5.0 * rng.standard_normal(N ** 3)).astype(np.float32)
+        ctx.set_field(0, noisy)
+        twin.set_field(0, noisy)
+        used, res = sur.advance(ctx, 0, 1e-3)
+        outcomes.append((used, res))
+        for _ in range(K):
+            twin.lod_substep(0)
+    log(f"[surrogate] noisy input: {outcomes}, {sur.gate_stats()}")
+    assert not outcomes[0][0] and not outcomes[1][0]
+    assert sur.gate_stats()["fallback"] == 1
+    assert np.array_equal(ctx.field(0).view(np.uint32), twin.field(0).view(np.uint32))
+    for o in (sur, ctx, twin):
+        o.close()
+    buf.free()
+    tbuf.free()
+
+
+def _bio_step(P, ctx, buf, n, sur, tol, step):
+    """One biology step with each run of K owed substeps handed to the
+    surrogate (the schedule bench.py's config N runs)."""
+    sur.reset_budget()
+    m = ctx.birth(buf, n, step)
+    owed = 0
+    for _ in range(rt.float_loop_count(P.dt_bio, P.dt_mech)):
+        ctx.fm_update(buf, m)
+        ctx.prelod(0, buf, m)
+        owed += rt.float_loop_count(P.dt_mech, P.dt_diff)
+        while owed >= K:
+            sur.advance(ctx, 0, tol)
+            owed -= K
+    for _ in range(owed):
+        ctx.lod_substep(0)
+    m2 = ctx.death(buf, m)
+    ctx.sync()
+    return m - n, m - m2, ctx.field(0)
+
+
+@pytest.mark.timeout(300)
+def test_default_tolerance_over_a_biology_step():
+    n = 200_000
+    P, ctx, buf = _state(n, 13, mech=61)
+    state = buf.download(rt.CELL_DTYPE, P.capacity)
+    rng = np.random.default_rng(4)
+    state["sign"][:n][rng.random(n) < 0.01] = 0  # dead cells for this step's death
+    field0 = ctx.field(0)
+    out = {}
+    for tol in (0.0, 1e-3):
+        buf.upload(state)
+        ctx.set_field(0, field0)
+        sur = rt.Surrogate(N, K)
+        sur.set_params(weights())
+        out[tol] = _bio_step(P, ctx, buf, n, sur, tol, 1) + (sur.gate_stats(),)
+        sur.close()
+    (b0, d0, f0, _), (b1, d1, f1, st) = out[0.0], out[1e-3]
+    err = rel_l2(f1, f0)
+    log(f"[surrogate] bio step: births {b0}/{b1}, deaths {d0}/{d1}, field rel L2 {err:.2e}, gate {st}")
+    assert b0 > 0 and d0 > 0
+    assert err <= 1e-3
+    assert abs(b1 - b0) <= 0.01 * b0 and abs(d1 - d0) <= 0.01 * d0
+    ctx.close()
+    buf.free()
+
+
+def test_network_beats_the_method_without_it_on_a_relaxing_field():
+    P, ctx, buf = _state(60_000, 14, profile=False, mech=0)
+    ctx.fm_update(buf, 60_000)
+    ctx.prelod(0, buf, 60_000)
+    ctx.sync()
+    p0 = ctx.field(0)
+    for _ in range(K):
+        ctx.lod_substep(0)
+    truth = ctx.field(0)
+    ctx.set_field(0, p0)
+    errs = {}
+    for name, w in (("trained", weights()), ("zero", np.zeros_like(weights()))):
+        sur = rt.Surrogate(N, K)
+        sur.set_params(w)
+        pred = rt.DeviceBuffer(N ** 3 * 4)
+        sur.predict(ctx.field_ptr(0), ctx.lib.gell_uptake_field(ctx.h, 0), pred)
+        ctx.sync()
+        errs[name] = rel_l2(pred.download(np.float32, N ** 3), truth)
+        pred.free()
+        sur.close()
+    log(f"[surrogate] relaxing field, 50 substeps: trained {errs['trained']:.2e}, without network "
+        f"{errs['zero']:.2e}, identity {rel_l2(p0, truth):.2e}")
+    assert abs(errs["zero"] - rel_l2(p0, truth)) <= 1e-7  # zero weights predict the input itself
+    assert errs["trained"] < 0.5 * errs["zero"]
+    ctx.close()
+    buf.free()
