@@ -146,6 +146,170 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Convolution of an upsampled tensor, on the coarse grid. y = conv3(up2(d))
+// with up2 the nearest-neighbour 2x upsampling of the S^3 tensor d: along
+// each axis fine voxel 2c + p (parity p) reads up2(d) at 2c + p + t - 1,
+// t = 0, 1, 2, which is coarse voxel c - 1 + p + e with e = (p + t + 1) / 2 - p
+// (0, 1, 1 for p = 0; 0, 0, 1 for p = 1), zero outside on both grids. So each
+// of the 8 parities is a 2x2x2 convolution of d whose weights are sums of
+// the original taps (combine_up_weights): 8 CIN products per output voxel
+// instead of 27 CIN, and no upsampled tensor.
+// Block = 4 x 4 x TZ coarse voxels (their halo, 6 x 6 x (TZ + 2) x CIN, in
+// LDS) for all 8 parities; a 16-voxel tile is 4 x 4 coarse voxels at one z
+// for one parity; wave w takes parities 2w and 2w + 1. K = 8 CIN, k = e3 CIN
+// + c, e3 = (ez 2 + ey) 2 + ex. Weights W'[p][o][k] (combine_up_weights).
+// Epilogue as conv3_fwd (bias, ReLU, BF16) at fine voxel 2c + p of the
+// (2S)^3 output; with `skip` the stored value is BF16(BF16(that) + skip),
+// the U-Net's skip connection rounded as a separate add would round it (the
+// skip values are loaded before the halo, so their latency overlaps it).
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int TZ, int CG>
+__global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restrict__ d, const uint16_t* __restrict__ w,
+                                                       const float* __restrict__ bias,
+                                                       const uint16_t* __restrict__ skip, uint16_t* __restrict__ y,
+                                                       int S, int relu) {
+    constexpr int HX = 6, HY = 6, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int CH = CIN / 8;
+    constexpr int KP = 8 * CIN;
+    constexpr int KSTEPS = KP / 32;
+    constexpr int NCT = cpad(COUT) / 16;
+    constexpr int VPW = 2 * TZ;  // two parities x TZ z slices
+    static_assert(CIN % 32 == 0 && NCT % CG == 0, "tile shape");
+    __shared__ uint4 halo[HV * CH];
+    const int zb = S / TZ;
+    const int bz = blockIdx.z % zb, ct0 = (blockIdx.z / zb) * CG;
+    const int cx0 = blockIdx.x * 4, cy0 = blockIdx.y * 4, cz0 = bz * TZ;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int lx = r & 3, ly = r >> 2;
+    const int F = 2 * S;
+    // tile a: parity 2 wv + (a & 1), coarse z slice a >> 1; its output voxel
+    // for this lane, and the skip values there, loaded before anything waits
+    int64_t vox[VPW];
+    uint2 skv[VPW][CG];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a) {
+        const int p = 2 * wv + (a & 1);
+        const int fx = 2 * (cx0 + lx) + (p & 1), fy = 2 * (cy0 + ly) + ((p >> 1) & 1),
+                  fz = 2 * (cz0 + (a >> 1)) + (p >> 2);
+        vox[a] = ((int64_t)fz * F + fy) * F + fx;
+#pragma unroll
+        for (int c = 0; c < CG; ++c) {
+            const int o = (ct0 + c) * 16 + 4 * g;
+            skv[a][c] = (skip && o < COUT) ? *reinterpret_cast<const uint2*>(skip + vox[a] * COUT + o)
+                                           : make_uint2(0u, 0u);
+        }
+    }
+    for (int q = threadIdx.x; q < HV * CH; q += kThreads) {
+        const int ch = q % CH, hv = q / CH;
+        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int gx = cx0 - 1 + hx, gy = cy0 - 1 + hy, gz = cz0 - 1 + hz;
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (gx >= 0 && gx < S && gy >= 0 && gy < S && gz >= 0 && gz < S)
+            val = *reinterpret_cast<const uint4*>(d + (((int64_t)gz * S + gy) * S + gx) * CIN + ch * 8);
+        halo[q] = val;
+    }
+    __syncthreads();
+    f4 acc[VPW][CG];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
+    // the halo voxel of window offset 0 for this lane's coarse voxel
+    int hbase[VPW];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a) {
+        const int p = 2 * wv + (a & 1);
+        const int px = p & 1, py = (p >> 1) & 1, pz = p >> 2;
+        hbase[a] = ((a >> 1) + pz) * HX * HY + (ly + py) * HX + (lx + px);
+    }
+    // A fragments of the wave's two parities, one step ahead
+    const uint16_t* wr0 = w + ((int64_t)(2 * wv) * cpad(COUT) + ct0 * 16 + r) * KP + g * 8;
+    const uint16_t* wr1 = wr0 + (int64_t)cpad(COUT) * KP;
+    bf8 af[2][CG], an[2][CG];
+#pragma unroll
+    for (int c = 0; c < CG; ++c) {
+        an[0][c] = *reinterpret_cast<const bf8*>(wr0 + c * 16 * KP);
+        an[1][c] = *reinterpret_cast<const bf8*>(wr1 + c * 16 * KP);
+    }
+    for (int s = 0; s < KSTEPS; ++s) {
+        const int k0 = s * 32 + g * 8;
+        const int e3 = k0 / CIN, c0 = k0 % CIN;
+        const int off = ((e3 >> 2) * HY + ((e3 >> 1) & 1)) * HX + (e3 & 1);
+#pragma unroll
+        for (int c = 0; c < CG; ++c) {
+            af[0][c] = an[0][c];
+            af[1][c] = an[1][c];
+        }
+        if (s + 1 < KSTEPS) {
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                an[0][c] = *reinterpret_cast<const bf8*>(wr0 + c * 16 * KP + (s + 1) * 32);
+                an[1][c] = *reinterpret_cast<const bf8*>(wr1 + c * 16 * KP + (s + 1) * 32);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < VPW; ++a) {
+            const bf8 bf = __builtin_bit_cast(bf8, halo[(hbase[a] + off) * CH + c0 / 8]);
+#pragma unroll
+            for (int c = 0; c < CG; ++c)
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a & 1][c], bf, acc[a][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < VPW; ++a) {
+#pragma unroll
+        for (int c = 0; c < CG; ++c) {
+            const int o = (ct0 + c) * 16 + 4 * g;
+            if (o >= COUT) continue;
+            uint16_t h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float t = acc[a][c][i] + bias[o + i];
+                if (relu && t < 0.f) t = 0.f;
+                h[i] = f2bf(t);
+            }
+            if (skip) {
+                const uint2 sk = skv[a][c];
+                const uint16_t e[4] = {(uint16_t)(sk.x & 0xffffu), (uint16_t)(sk.x >> 16), (uint16_t)(sk.y & 0xffffu),
+                                       (uint16_t)(sk.y >> 16)};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) h[i] = f2bf(bf2f(h[i]) + bf2f(e[i]));
+            }
+            uint2 pk;
+            pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+            pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+            *reinterpret_cast<uint2*>(y + vox[a] * COUT + o) = pk;
+        }
+    }
+}
+
+// W'[p][o][e3 CIN + c] = BF16(sum of the FP32 master weights m[o][c][tap]
+// over the taps t with e(t) = e3 on every axis), for the 8 parities p; rows
+// o >= cout are zero.
+__global__ void combine_up_weights(const float* __restrict__ m, int cin, int cout, uint16_t* __restrict__ w) {
+    const int coutp = cpad(cout), kp = 8 * cin;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)8 * coutp * kp) return;
+    const int p = (int)(i / ((int64_t)coutp * kp));
+    const int o = (int)((i / kp) % coutp), k = (int)(i % kp);
+    const int e3 = k / cin, c = k % cin;
+    const int pp[3] = {p & 1, (p >> 1) & 1, p >> 2}, ee[3] = {e3 & 1, (e3 >> 1) & 1, e3 >> 2};
+    float sum = 0.f;
+    if (o < cout) {
+        for (int tz = 0; tz < 3; ++tz)
+            for (int ty = 0; ty < 3; ++ty)
+                for (int tx = 0; tx < 3; ++tx) {
+                    const int t[3] = {tx, ty, tz};
+                    bool hit = true;
+                    for (int a = 0; a < 3; ++a) hit = hit && ((pp[a] + t[a] + 1) / 2 - pp[a] == ee[a]);
+                    if (hit) sum += m[((int64_t)o * cin + c) * 27 + (tz * 3 + ty) * 3 + tx];
+                }
+    }
+    w[i] = f2bf(sum);
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient dW[o][tap CIN + c] = sum_v dy[v][o] x[v + d(tap)][c] over
 // one block of 8 x 8 x TZ voxels, as MFMAs with the voxels on the K axis:
 // A = dy^T (16 channels x 32 voxels), B = shifted x (32 voxels x 16 k). The
@@ -387,36 +551,119 @@ __global__ void add_bf(const uint16_t* __restrict__ a, const uint16_t* __restric
     reinterpret_cast<uint4*>(out)[i] = pack8(fa);
 }
 
+// The increment r = p1 - p0 of one real substep (p1 = the field after one
+// LOD substep from p0 with the uptake map held fixed) and its scale. Within
+// one call the substep is affine in the field (Setboundary, the implicit
+// uptake p / (1 - rate dt) and the three Thomas sweeps), so the field after
+// K substeps is p0 + (1 + A + ... + A^(K-1)) r with A the substep's linear
+// part: a linear map of r that depends only on the uptake map. The network
+// sees r in units of its interior RMS s and predicts the K-substep increment
+// in the same units, so its error scales with how far the field moves.
+//
+// incr_sums: per-block partial sums (double) of r^2 over the interior (the
+// voxels two or more from every face) and of p0^2 over the grid. Each
+// thread takes 4 consecutive voxels of an x line (N is a multiple of 128, so
+// N^3 / 4 < 2^31 up to N = 1024).
+__global__ __launch_bounds__(256) void incr_sums(const float* __restrict__ p0, const float* __restrict__ p1, int N,
+                                                 double* __restrict__ part) {
+    __shared__ double r0[256], r1[256];
+    const uint32_t q = (uint32_t)N / 4u, nq = (uint32_t)N * (uint32_t)N * q;
+    double s0 = 0.0, s1 = 0.0;
+    for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < nq; c += gridDim.x * 256u) {
+        const uint32_t line = c / q, x0 = (c - line * q) * 4u;
+        const uint32_t y = line % (uint32_t)N, z = line / (uint32_t)N;
+        const float4 a = reinterpret_cast<const float4*>(p0)[c];
+        const float4 b = reinterpret_cast<const float4*>(p1)[c];
+        const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+        const bool row_in = y >= 2u && z >= 2u && y < (uint32_t)N - 2u && z < (uint32_t)N - 2u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double ad = (double)av[k];
+            s1 += ad * ad;
+            const uint32_t x = x0 + (uint32_t)k;
+            if (row_in && x >= 2u && x < (uint32_t)N - 2u) {
+                const double d = (double)bv[k] - ad;
+                s0 += d * d;
+            }
+        }
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
+// One block, fixed order: norm[0] = s = sqrt(sum r^2 / interior count),
+// norm[1] = 1 / s (0 when s = 0: then the prediction is p0 itself);
+// sums[0..1] = the two sums.
+__global__ __launch_bounds__(256) void incr_finalize(const double* __restrict__ part, int nblk, double inv_count,
+                                                     float* __restrict__ norm, double* __restrict__ sums) {
+    __shared__ double r0[256], r1[256];
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        s0 += part[2 * b];
+        s1 += part[2 * b + 1];
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double s = sqrt(r0[0] * inv_count);
+        norm[0] = (float)s;
+        norm[1] = s > 0.0 ? (float)(1.0 / s) : 0.f;
+        sums[0] = r0[0];
+        sums[1] = r1[0];
+    }
+}
+
 // Input features on the half-resolution grid (X = N / 2), 24 channels per
-// coarse voxel: q = p / bnd - 1 at its 8 fine voxels split into a BF16 high
-// part (channels 0..7) and the BF16 rounding of the remainder (8..15), so the
-// pair carries ~16 significant bits; then the uptake rate times rate_scale at
-// the same 8 (16..23). Fine voxel d = (z & 1) 4 + (y & 1) 2 + (x & 1).
-__global__ void features(const float* __restrict__ p, const float* __restrict__ rate, int N, float inv_bnd,
-                         float rate_scale, uint16_t* __restrict__ f) {
+// coarse voxel: q = (p1 - p0) / s at its 8 fine voxels split into a BF16 high
+// part (channels 0..7) and the BF16 rounding of the remainder (8..15), then
+// the uptake rate times rate_scale at the same 8 (16..23). Fine voxel
+// d = (z & 1) 4 + (y & 1) 2 + (x & 1).
+__global__ void features(const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ rate,
+                         int N, const float* __restrict__ norm, float rate_scale, uint16_t* __restrict__ f) {
     const int X = N / 2;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (int64_t)X * X * X) return;
     const int x = (int)(v % X), y = (int)((v / X) % X), z = (int)(v / ((int64_t)X * X));
+    const float inv_s = norm[1];
     // the fine voxels come in x pairs: four 8-byte loads per field
-    float pv[8], rv[8];
+    float qv[8], rv[8];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int yi = 2 * y + (q & 1), zi = 2 * z + (q >> 1);
         const int64_t o = ((int64_t)zi * N + yi) * N + 2 * x;
-        const float2 a = *reinterpret_cast<const float2*>(p + o);
+        const float2 a = *reinterpret_cast<const float2*>(p0 + o);
+        const float2 b = *reinterpret_cast<const float2*>(p1 + o);
         const float2 r = *reinterpret_cast<const float2*>(rate + o);
-        pv[2 * q] = a.x;
-        pv[2 * q + 1] = a.y;
+        qv[2 * q] = (b.x - a.x) * inv_s;
+        qv[2 * q + 1] = (b.y - a.y) * inv_s;
         rv[2 * q] = r.x;
         rv[2 * q + 1] = r.y;
     }
     uint16_t h[24];
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-        const float q = pv[d] * inv_bnd - 1.f;
-        h[d] = f2bf(q);
-        h[8 + d] = f2bf(q - bf2f(h[d]));
+        h[d] = f2bf(qv[d]);
+        h[8 + d] = f2bf(qv[d] - bf2f(h[d]));
         h[16 + d] = f2bf(rv[d] * rate_scale);
     }
     uint4 pk[3];
@@ -431,42 +678,47 @@ __global__ void features(const float* __restrict__ p, const float* __restrict__ 
     for (int q = 0; q < 3; ++q) reinterpret_cast<uint4*>(f + v * 24)[q] = pk[q];
 }
 
-// Full-resolution output: out[fine] = p[fine] + out_scale * head[coarse][sub],
-// sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1); the two outer layers, which every
-// substep resets, keep p. One thread per coarse voxel: its 8 head values in
-// one 16-byte load, its fine voxels as 4 x-pairs.
-__global__ void head_out(const float* __restrict__ p, const uint16_t* __restrict__ head, int N, float out_scale,
-                         float bnd, float* __restrict__ out) {
+// Full-resolution output: out[fine] = p0[fine] + s out_scale head[coarse][sub],
+// sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1), in the interior; the two outer
+// layers, which every substep first resets to the boundary value, take p1
+// (out may alias p1). One thread per coarse voxel: its 8 head values in one
+// 16-byte load, its fine voxels as 4 x-pairs.
+__global__ void head_out(const float* __restrict__ p0, const float* p1, const uint16_t* __restrict__ head, int N,
+                         const float* __restrict__ norm, float out_scale, float* out) {
     const int X = N / 2;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (int64_t)X * X * X) return;
     const int x = (int)(c % X), y = (int)((c / X) % X), z = (int)(c / ((int64_t)X * X));
+    const float scale = norm[0] * out_scale;
     float h[8];
     unpack8(*reinterpret_cast<const uint4*>(head + c * 8), h);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int yi = 2 * y + (q & 1), zi = 2 * z + (q >> 1);
         const int64_t o = ((int64_t)zi * N + yi) * N + 2 * x;
-        const float2 pv = *reinterpret_cast<const float2*>(p + o);
+        const float2 a = *reinterpret_cast<const float2*>(p0 + o);
+        const float2 b = *reinterpret_cast<const float2*>(p1 + o);
         const bool row_edge = yi < 2 || zi < 2 || yi >= N - 2 || zi >= N - 2;
         const int x0 = 2 * x, x1 = 2 * x + 1;
         float2 r;
-        r.x = (row_edge || x0 < 2 || x0 >= N - 2) ? pv.x : pv.x + out_scale * h[q * 2];
-        r.y = (row_edge || x1 < 2 || x1 >= N - 2) ? pv.y : pv.y + out_scale * h[q * 2 + 1];
+        r.x = (row_edge || x0 < 2 || x0 >= N - 2) ? b.x : a.x + scale * h[q * 2];
+        r.y = (row_edge || x1 < 2 || x1 >= N - 2) ? b.y : a.y + scale * h[q * 2 + 1];
         *reinterpret_cast<float2*>(out + o) = r;
     }
 }
 
-// Loss = mean over interior fine voxels of (pred - target)^2 / scale^2, with
-// pred = p + out_scale head; writes dL/dhead (bf16, coarse x 8) and the
-// per-block partial sums of the loss.
-__global__ __launch_bounds__(256) void mse_head(const float* __restrict__ p, const uint16_t* __restrict__ head,
-                                                const float* __restrict__ target, int N, float out_scale,
-                                                float inv_count, uint16_t* __restrict__ dhead,
-                                                float* __restrict__ loss_part) {
+// Loss = mean over interior fine voxels of (head - (target - p0) / (s
+// out_scale))^2: the squared error of the predicted increment in units of s
+// out_scale; writes dL/dhead (bf16, coarse x 8) and the per-block partial
+// sums of the loss.
+__global__ __launch_bounds__(256) void mse_head(const float* __restrict__ p0, const uint16_t* __restrict__ head,
+                                                const float* __restrict__ target, int N,
+                                                const float* __restrict__ norm, float out_scale, float inv_count,
+                                                uint16_t* __restrict__ dhead, float* __restrict__ loss_part) {
     __shared__ float red[256];
     const int X = N / 2;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // coarse voxel
+    const float inv = norm[1] / out_scale;
     float l = 0.f;
     if (c < (int64_t)X * X * X) {
         const int x = (int)(c % X), y = (int)((c / X) % X), z = (int)(c / ((int64_t)X * X));
@@ -476,8 +728,7 @@ __global__ __launch_bounds__(256) void mse_head(const float* __restrict__ p, con
             const int64_t o = ((int64_t)zi * N + yi) * N + xi;
             const bool edge = xi < 2 || yi < 2 || zi < 2 || xi >= N - 2 || yi >= N - 2 || zi >= N - 2;
             const float h = bf2f(head[c * 8 + sub]);
-            // in units of out_scale: e = head - (target - p) / out_scale
-            const float e = edge ? 0.f : h - (target[o] - p[o]) / out_scale;
+            const float e = edge ? 0.f : h - (target[o] - p0[o]) * inv;
             l += e * e;
             gh[sub] = f2bf(2.f * e * inv_count);
         }
@@ -541,6 +792,25 @@ hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t
     return hipGetLastError();
 }
 
+// Coarse z slices per up-convolution block: the halo stays within ~37 KB.
+constexpr int up_tz(int cin) { return cin >= 128 ? 2 : 4; }
+
+template <int CIN, int COUT>
+hipError_t upconv_t(const uint16_t* d, const uint16_t* w, const float* b, const uint16_t* skip, uint16_t* y, int S,
+                    int relu, hipStream_t st) {
+    constexpr int TZ = up_tz(CIN);
+    constexpr int NCT = cpad(COUT) / 16;
+    if (S % 4 || S % TZ) return hipErrorInvalidValue;
+    const int64_t base = (int64_t)(S / 4) * (S / 4) * (S / TZ);
+    if (base >= 256 || NCT == 1)
+        hipLaunchKernelGGL((upconv_fwd<CIN, COUT, TZ, NCT>), dim3(S / 4, S / 4, S / TZ), dim3(kThreads), 0, st, d, w, b,
+                           skip, y, S, relu);
+    else
+        hipLaunchKernelGGL((upconv_fwd<CIN, COUT, TZ, 1>), dim3(S / 4, S / 4, (S / TZ) * NCT), dim3(kThreads), 0, st,
+                           d, w, b, skip, y, S, relu);
+    return hipGetLastError();
+}
+
 template <int CIN, int COUT>
 hipError_t wgrad_t(const uint16_t* x, const uint16_t* dy, float* part, float* out, int X, int Y, int Z, int accumulate,
                    hipStream_t st) {
@@ -569,6 +839,22 @@ hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t*
     GELL_NN_CASE(32, 64) GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64)
     GELL_NN_CASE(64, 32) GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8) GELL_NN_CASE(8, 16)
 #undef GELL_NN_CASE
+    return hipErrorInvalidValue;
+}
+
+int64_t up_weight_count(int cin, int cout) { return (int64_t)8 * cpad(cout) * 8 * cin; }
+
+hipError_t up_weights(const float* m, int cin, int cout, uint16_t* w8, hipStream_t st) {
+    hipLaunchKernelGGL(combine_up_weights, dim3(blocks_for(up_weight_count(cin, cout))), dim3(256), 0, st, m, cin, cout,
+                       w8);
+    return hipGetLastError();
+}
+
+hipError_t upconv3(const uint16_t* d, const uint16_t* w8, const float* b, uint16_t* y, int cin, int cout, int S,
+                   int relu, hipStream_t st, const uint16_t* skip) {
+    if (cin == 128 && cout == 64) return upconv_t<128, 64>(d, w8, b, skip, y, S, relu, st);
+    if (cin == 64 && cout == 32) return upconv_t<64, 32>(d, w8, b, skip, y, S, relu, st);
+    if (cin == 32 && cout == 16) return upconv_t<32, 16>(d, w8, b, skip, y, S, relu, st);
     return hipErrorInvalidValue;
 }
 
@@ -637,26 +923,35 @@ hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, h
     return hipGetLastError();
 }
 
-hipError_t input_features(const float* p, const float* rate, int N, float bnd, float rate_scale, uint16_t* f,
-                          hipStream_t st) {
+hipError_t increment_norms(const float* p0, const float* p1, int N, double* part, float* norm, double* sums,
+                           hipStream_t st) {
+    const int64_t interior = (int64_t)(N - 4) * (N - 4) * (N - 4);
+    hipLaunchKernelGGL(incr_sums, dim3(kNormBlocks), dim3(256), 0, st, p0, p1, N, part);
+    hipLaunchKernelGGL(incr_finalize, dim3(1), dim3(256), 0, st, part, kNormBlocks, 1.0 / (double)interior, norm,
+                       sums);
+    return hipGetLastError();
+}
+
+hipError_t input_features(const float* p0, const float* p1, const float* rate, int N, const float* norm,
+                          float rate_scale, uint16_t* f, hipStream_t st) {
     const int64_t n = (int64_t)(N / 2) * (N / 2) * (N / 2);
-    hipLaunchKernelGGL(features, dim3(blocks_for(n)), dim3(256), 0, st, p, rate, N, 1.f / bnd, rate_scale, f);
+    hipLaunchKernelGGL(features, dim3(blocks_for(n)), dim3(256), 0, st, p0, p1, rate, N, norm, rate_scale, f);
     return hipGetLastError();
 }
 
-hipError_t output_field(const float* p, const uint16_t* head, int N, float out_scale, float bnd, float* out,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)(N / 2) * (N / 2) * (N / 2))), dim3(256), 0, st, p, head, N,
-                       out_scale, bnd, out);
+hipError_t output_field(const float* p0, const float* p1, const uint16_t* head, int N, const float* norm,
+                        float out_scale, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)(N / 2) * (N / 2) * (N / 2))), dim3(256), 0, st, p0, p1,
+                       head, N, norm, out_scale, out);
     return hipGetLastError();
 }
 
-hipError_t loss_head(const float* p, const uint16_t* head, const float* target, int N, float out_scale,
-                     uint16_t* dhead, float* loss_part, int64_t* nparts, hipStream_t st) {
+hipError_t loss_head(const float* p0, const uint16_t* head, const float* target, int N, const float* norm,
+                     float out_scale, uint16_t* dhead, float* loss_part, int64_t* nparts, hipStream_t st) {
     const int64_t nc = (int64_t)(N / 2) * (N / 2) * (N / 2);
     const int64_t interior = (int64_t)(N - 4) * (N - 4) * (N - 4);
     *nparts = blocks_for(nc);
-    hipLaunchKernelGGL(mse_head, dim3((unsigned)*nparts), dim3(256), 0, st, p, head, target, N, out_scale,
+    hipLaunchKernelGGL(mse_head, dim3((unsigned)*nparts), dim3(256), 0, st, p0, head, target, N, norm, out_scale,
                        1.f / (float)interior, dhead, loss_part);
     return hipGetLastError();
 }

@@ -1,11 +1,23 @@
 // surrogate.hip -- a learned stand-in for K LOD substeps with a fixed uptake
 // map (no counterpart in the reference: its time loop always runs every
-// substep, Main.cu:121-123). The network, on the half-resolution grid
-// (N/2)^3, is a four-level U-Net of 3x3x3 convolutions (BF16 on the matrix
-// cores, FP32 accumulation):
+// substep, Main.cu:121-123).
 //
-//   features(p, rate) 24 ch: p (as BF16 high + low parts) and rate at the 8
-//                   fine voxels of each coarse one
+// What it predicts: with the uptake map fixed, one substep is affine in the
+// field, p -> A p + c (Setboundary, the implicit uptake p / (1 - rate dt),
+// the three Thomas sweeps; LODkernel.cuh:24-136). So the field after K
+// substeps from p0 is p0 + (1 + A + ... + A^(K-1)) r, r = p1 - p0 being the
+// increment of the first substep: a linear map of r that depends only on the
+// uptake map. The network takes r in units of its interior RMS s together
+// with the uptake map and returns the K-substep increment in units of
+// s out_scale; the prediction is p0 + s out_scale head. Its error thus scales
+// with how far the field actually moves: near a quasi-steady state, where the
+// K substeps change little, so does the prediction.
+//
+// The network, on the half-resolution grid (N/2)^3, is a four-level U-Net of
+// 3x3x3 convolutions (BF16 on the matrix cores, FP32 accumulation):
+//
+//   features(r / s, rate) 24 ch: r / s (as BF16 high + low parts) and rate
+//                   at the 8 fine voxels of each coarse one
 //   level 1 (N/2):  L0 24->16, L1 16->16 = e1             pool
 //   level 2 (N/4):  L2 16->32, L3 32->32 = e2             pool
 //   level 3 (N/8):  L4 32->64, L5 64->64 = e3             pool
@@ -14,8 +26,8 @@
 //   level 2:        L10 64->32 (+ e2), L11 32->32 = d2    up
 //   level 1:        L12 32->16 (+ e1), L13 16->16 = d1
 //   head:           L14 16->8, no ReLU: the 2x2x2 fine voxels of each coarse
-//                   voxel, added to p (in units of out_scale); the two outer
-//                   layers keep the boundary value.
+//                   voxel, added to p0 (in units of s out_scale); the two
+//                   outer layers, which every substep resets first, take p1.
 //
 // Every layer but the head applies ReLU; the "+ e" skips are added after it.
 // Training (gradients + Adam) runs on the same kernels: input gradients are
@@ -63,6 +75,7 @@ struct gell_surrogate {
     float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     uint16_t* wb[kNL] = {};  // GEMM-layout BF16 weights
     uint16_t* wf[kNL] = {};  // flipped, for input gradients
+    uint16_t* wu[kNL] = {};  // the layers after an upsampling: per-parity 2x2x2 weights
     bool packed = false;     // wb / wf current
     float* zero_bias = nullptr;
     // forward tensors
@@ -74,15 +87,20 @@ struct gell_surrogate {
     float* wg_out = nullptr;
     float* loss_part = nullptr;
     float* h_red = nullptr;  // pinned
-    float* pred = nullptr;   // N^3 scratch for advance
-    float* check = nullptr;  // N^3 scratch for advance
+    float* pred = nullptr;   // N^3 scratch for advance: the prediction at a check
+    float* p0 = nullptr;     // N^3 scratch for advance: the field before the call
     float* red_part = nullptr;
+    double* norm_part = nullptr;  // increment_norms partials
+    float* norm = nullptr;        // s, 1 / s (device)
+    double* sums = nullptr;       // sum r^2 (interior), sum p0^2 (device)
+    double* h_sums = nullptr;     // pinned copy of sums
     int adam_t = 0;
     int accumulate = 0;  // add this call's gradients to the stored ones
     // The in-loop gate (gell_surrogate_advance): the largest measured ratio
-    // of a prediction's true error to its one-substep residual (< 0: none
-    // measured yet), the calls since the last exact check, the error spent in
-    // the current budget window, and counts of each kind of call.
+    // of a prediction's true error to the relative size of its first
+    // substep's increment (< 0: none measured yet), the calls since the last
+    // exact check, the error spent in the current budget window, and counts
+    // of each kind of call.
     double ratio = -1.0;
     int since_check = 0;
     double spent = 0.0;
@@ -120,11 +138,15 @@ hipError_t tensor(gell_surrogate* s, Tensor& t, int S, int C) {
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
+// The layers whose input is an upsampled tensor (8, 10, 12).
+bool after_up(int l) { return l == 8 || l == 10 || l == 12; }
+
 hipError_t pack_all(gell_surrogate* s) {
     for (int l = 0; l < kNL; ++l) {
         const LayerSpec& L = kLayers[l];
         SCHK(weights_pack(s->param + s->w_off[l], L.cin, L.cout, s->wb[l], s->stream));
         SCHK(weights_flip(s->wb[l], L.cin, L.cout, s->wf[l], s->stream));
+        if (after_up(l)) SCHK(up_weights(s->param + s->w_off[l], L.cin, L.cout, s->wu[l], s->stream));
     }
     s->packed = true;
     return hipSuccess;
@@ -136,10 +158,26 @@ hipError_t conv_layer(gell_surrogate* s, int l, const Tensor& in, Tensor& out) {
                  s->stream);
 }
 
-hipError_t forward(gell_surrogate* s, const float* p, const float* rate) {
+// out = layer l (one of 8, 10, 12) of up2(coarse) + skip, on the coarse grid.
+hipError_t up_layer(gell_surrogate* s, int l, const Tensor& coarse, Tensor& out, const uint16_t* skip) {
+    const LayerSpec& L = kLayers[l];
+    return upconv3(coarse.d, s->wu[l], s->param + s->b_off[l], out.d, L.cin, L.cout, coarse.S, L.relu ? 1 : 0,
+                   s->stream, skip);
+}
+
+// The increment's norms (s on the device) and the network's head from
+// (p0, p1 = one substep from p0, rate). With train each upsampling is formed
+// (u3, u2, u1) and the skip connections keep the convolution's output (c3,
+// c2, c1: the backward pass needs its ReLU mask) apart from the sum; without,
+// the layers after an upsampling run on the coarse grid (upconv3: its
+// per-parity weights are sums of the layer's FP32 taps rounded to BF16 once,
+// where the formed path rounds each tap, a difference of one BF16 rounding
+// per weight) and store the sum themselves.
+hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const float* rate, bool train) {
     if (!s->packed) SCHK(pack_all(s));
     hipStream_t st = s->stream;
-    SCHK(input_features(p, rate, s->N, s->cfg.boundary, s->cfg.rate_scale, s->f0.d, st));
+    SCHK(increment_norms(p0, p1, s->N, s->norm_part, s->norm, s->sums, st));
+    SCHK(input_features(p0, p1, rate, s->N, s->norm, s->cfg.rate_scale, s->f0.d, st));
     SCHK(conv_layer(s, 0, s->f0, s->a0));
     SCHK(conv_layer(s, 1, s->a0, s->e1));
     SCHK(pool(s->e1.d, s->p1.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
@@ -151,17 +189,29 @@ hipError_t forward(gell_surrogate* s, const float* p, const float* rate) {
     SCHK(pool(s->e3.d, s->p3.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));
     SCHK(conv_layer(s, 6, s->p3, s->a6));
     SCHK(conv_layer(s, 7, s->a6, s->e4));
-    SCHK(upsample(s->e4.d, s->u3.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));
-    SCHK(conv_layer(s, 8, s->u3, s->c3));
-    SCHK(add(s->c3.d, s->e3.d, s->x3.d, s->x3.n(), st));
+    if (train) {
+        SCHK(upsample(s->e4.d, s->u3.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));
+        SCHK(conv_layer(s, 8, s->u3, s->c3));
+        SCHK(add(s->c3.d, s->e3.d, s->x3.d, s->x3.n(), st));
+    } else {
+        SCHK(up_layer(s, 8, s->e4, s->x3, s->e3.d));
+    }
     SCHK(conv_layer(s, 9, s->x3, s->d3));
-    SCHK(upsample(s->d3.d, s->u2.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));
-    SCHK(conv_layer(s, 10, s->u2, s->c2));
-    SCHK(add(s->c2.d, s->e2.d, s->x2.d, s->x2.n(), st));
+    if (train) {
+        SCHK(upsample(s->d3.d, s->u2.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));
+        SCHK(conv_layer(s, 10, s->u2, s->c2));
+        SCHK(add(s->c2.d, s->e2.d, s->x2.d, s->x2.n(), st));
+    } else {
+        SCHK(up_layer(s, 10, s->d3, s->x2, s->e2.d));
+    }
     SCHK(conv_layer(s, 11, s->x2, s->d2));
-    SCHK(upsample(s->d2.d, s->u1.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));
-    SCHK(conv_layer(s, 12, s->u1, s->c1));
-    SCHK(add(s->c1.d, s->e1.d, s->x1.d, s->x1.n(), st));
+    if (train) {
+        SCHK(upsample(s->d2.d, s->u1.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));
+        SCHK(conv_layer(s, 12, s->u1, s->c1));
+        SCHK(add(s->c1.d, s->e1.d, s->x1.d, s->x1.n(), st));
+    } else {
+        SCHK(up_layer(s, 12, s->d2, s->x1, s->e1.d));
+    }
     SCHK(conv_layer(s, 13, s->x1, s->d1));
     SCHK(conv_layer(s, 14, s->d1, s->h));
     return hipSuccess;
@@ -269,9 +319,8 @@ void gell_surrogate_default_config(gell_surrogate_config* c) {
     std::memset(c, 0, sizeof(*c));
     c->N = 256;
     c->K = 50;
-    c->boundary = 38.f;
     c->rate_scale = 0.1f;
-    c->out_scale = 1.f;
+    c->out_scale = 8.f;
     c->check_every = 6;
     c->seed = 7;
 }
@@ -282,7 +331,7 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     if (!cfg) return sfail(nullptr, GELL_E_ARG, "null config");
     if (cfg->N < 128 || cfg->N % 128 != 0 || cfg->N > 1024)
         return sfail(nullptr, GELL_E_UNSUPPORTED, "surrogate grid must be a multiple of 128 (128 .. 1024)");
-    if (cfg->K < 1 || !(cfg->boundary > 0.f) || !(cfg->out_scale > 0.f))
+    if (cfg->K < 1 || !(cfg->out_scale > 0.f))
         return sfail(nullptr, GELL_E_ARG, "bad surrogate config");
     gell_surrogate* s = new (std::nothrow) gell_surrogate;
     if (!s) return sfail(nullptr, GELL_E_NOMEM, "surrogate allocation");
@@ -308,6 +357,7 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     for (int l = 0; l < kNL; ++l) {
         SA(salloc(s, &s->wb[l], (int64_t)cpad(kLayers[l].cout) * kpad(kLayers[l].cin)));
         SA(salloc(s, &s->wf[l], (int64_t)cpad(kLayers[l].cin) * kpad(kLayers[l].cout)));
+        if (after_up(l)) SA(salloc(s, &s->wu[l], up_weight_count(kLayers[l].cin, kLayers[l].cout)));
     }
     SA(salloc(s, &s->zero_bias, 128));
     SA(tensor(s, s->f0, S1, 24));
@@ -360,9 +410,13 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     SA(salloc(s, &s->wg_out, wgo));
     SA(salloc(s, &s->loss_part, (int64_t)S1 * S1 * S1 / 256 + 1));
     SA(salloc(s, &s->pred, (int64_t)s->N * s->N * s->N));
-    SA(salloc(s, &s->check, (int64_t)s->N * s->N * s->N));
+    SA(salloc(s, &s->p0, (int64_t)s->N * s->N * s->N));
     SA(salloc(s, &s->red_part, 2 * kRedBlocks));
+    SA(salloc(s, &s->norm_part, 2 * kNormBlocks));
+    SA(salloc(s, &s->norm, 2));
+    SA(salloc(s, &s->sums, 2));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_red), sizeof(float) * (2 * kRedBlocks + 8));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_sums), sizeof(double) * 2);
 #undef SA
     if (e == hipSuccess) {
         // He-normal initialisation from a seeded host generator; the head
@@ -395,6 +449,7 @@ void gell_surrogate_destroy(gell_surrogate* s) {
     (void)hipStreamSynchronize(s->stream);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->h_red) (void)hipHostFree(s->h_red);
+    if (s->h_sums) (void)hipHostFree(s->h_sums);
     delete s;
 }
 
@@ -421,21 +476,23 @@ int gell_surrogate_set_params(gell_surrogate* s, const float* host, int64_t n) {
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, hipGetErrorString(e));
 }
 
-int gell_surrogate_predict(gell_surrogate* s, const float* field, const float* rate, float* out) {
-    if (!s || !field || !rate || !out) return sfail(s, GELL_E_ARG, "null argument");
-    hipError_t e = forward(s, field, rate);
-    if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, out, s->stream);
+int gell_surrogate_predict(gell_surrogate* s, const float* field, const float* field1, const float* rate, float* out) {
+    if (!s || !field || !field1 || !rate || !out) return sfail(s, GELL_E_ARG, "null argument");
+    hipError_t e = forward(s, field, field1, rate, false);
+    if (e == hipSuccess)
+        e = output_field(field, field1, s->h.d, s->N, s->norm, s->cfg.out_scale, out, s->stream);
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("predict: ") + hipGetErrorString(e));
 }
 
-int gell_surrogate_gradients(gell_surrogate* s, const float* field, const float* rate, const float* target,
-                             int32_t accumulate, float* loss, float* grad_host) {
-    if (!s || !field || !rate || !target) return sfail(s, GELL_E_ARG, "null argument");
+int gell_surrogate_gradients(gell_surrogate* s, const float* field, const float* field1, const float* rate,
+                             const float* target, int32_t accumulate, float* loss, float* grad_host) {
+    if (!s || !field || !field1 || !rate || !target) return sfail(s, GELL_E_ARG, "null argument");
     s->accumulate = accumulate ? 1 : 0;
-    hipError_t e = forward(s, field, rate);
+    hipError_t e = forward(s, field, field1, rate, true);
     int64_t np = 0;
     if (e == hipSuccess)
-        e = loss_head(field, s->h.d, target, s->N, s->cfg.out_scale, s->dh.d, s->loss_part, &np, s->stream);
+        e = loss_head(field, s->h.d, target, s->N, s->norm, s->cfg.out_scale, s->dh.d, s->loss_part, &np,
+                      s->stream);
     if (e == hipSuccess) e = backward(s);
     if (e == hipSuccess && loss) {
         std::vector<float> lp(np);
@@ -481,18 +538,19 @@ hipError_t rel_change(gell_surrogate* s, const float* a, const float* b, int64_t
 
 }  // namespace
 
-// The gate. Every call predicts and measures the one-substep residual r of
-// the prediction (its relative change under one real substep). On the first
-// call and every check_every-th call after, the K real substeps run as well
-// and the field keeps their exact result; the prediction's true relative
-// error e against them calibrates ratio = max(e / r) over the checks (times
-// a safety factor). Between checks a prediction is kept only when the error
-// budget allows it: its estimated error ratio * r, added to the estimates of
-// the predictions already kept in the current window
-// (gell_surrogate_reset_budget starts a new one), stays within tol; otherwise
-// the K real substeps run. The error a window adds to the field is thus
-// bounded by tol, as far as the calibrated estimate holds; with tol = 0
-// every step is exact.
+// The gate. Every call first runs one real substep (the first of the K) and
+// measures the relative size of its increment, rho = ||p1 - p0|| (interior)
+// / ||p0||. On the first call and every check_every-th call after, the
+// network predicts and the other K - 1 real substeps run as well; the field
+// keeps their exact result, and the prediction's true relative L2 error e
+// against it calibrates ratio = max(e / rho) over the checks. Between checks
+// the estimate 1.5 ratio rho decides before the network runs: the prediction
+// is made and kept when that estimate, added to the estimates of the
+// predictions already kept in the current window (gell_surrogate_reset_budget
+// starts a new one), stays within tol; otherwise the other K - 1 substeps
+// run and the network does not. The error a window adds to the field is thus
+// bounded by tol as far as the calibrated estimate holds; tol = 0 makes every
+// call exact.
 int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float tol, int32_t* used, float* residual) {
     if (!s || !ctx || !used) return sfail(s, GELL_E_ARG, "null argument");
     if (ctx->N != s->N) return sfail(s, GELL_E_ARG, "grid size differs from the surrogate's");
@@ -502,43 +560,44 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     const int64_t V = ctx->V;
     float* const field = ctx->field[sub];
     *used = 0;
+    // keep p0, run the first substep for real (field = p1), measure its increment
     hipError_t e = gell::join_substrates(ctx);
-    if (e == hipSuccess) e = forward(s, field, ctx->rate[sub]);
-    if (e == hipSuccess) e = output_field(field, s->h.d, s->N, s->cfg.out_scale, s->cfg.boundary, s->pred, st);
-    // one more substep from the prediction: its relative change is the residual
-    if (e == hipSuccess) e = hipMemcpyAsync(s->check, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) {
-        ctx->field[sub] = s->check;
-        e = gell::lod_substep_launch(ctx, sub, st);
-        ctx->field[sub] = field;
-    }
-    double res = 0.0;
-    if (e == hipSuccess) e = rel_change(s, s->check, s->pred, V, &res);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->p0, field, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = gell::lod_substep_launch(ctx, sub, st);
+    if (e == hipSuccess) e = increment_norms(s->p0, field, s->N, s->norm_part, s->norm, s->sums, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->h_sums, s->sums, sizeof(double) * 2, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
-    if (residual) *residual = (float)res;
+    const double rho = s->h_sums[1] > 0.0 ? std::sqrt(s->h_sums[0] / s->h_sums[1]) : 0.0;
+    if (residual) *residual = (float)rho;
     const int every = s->cfg.check_every > 0 ? s->cfg.check_every : 1;
     if (s->ratio < 0.0 || s->since_check + 1 >= every) {
-        // exact check: the K real substeps, kept; the prediction's error calibrates the estimate
-        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
+        // exact check: predict, then the other K - 1 real substeps, kept; the
+        // prediction's error calibrates the estimate
+        e = forward(s, s->p0, field, ctx->rate[sub], false);
+        if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, s->pred, st);
+        for (int k = 1; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
         double err = 0.0;
         if (e == hipSuccess) e = rel_change(s, field, s->pred, V, &err);
         if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance check: ") + hipGetErrorString(e));
-        const double r = err / (res > 1e-30 ? res : 1e-30);
+        const double r = err / (rho > 1e-30 ? rho : 1e-30);
         s->ratio = r > s->ratio ? r : s->ratio;
         s->since_check = 0;
         ++s->n_checked;
         return GELL_OK;
     }
     ++s->since_check;
-    const double est = 1.5 * s->ratio * res;
-    if (s->spent + est <= (double)tol) {
+    const double est = 1.5 * s->ratio * rho;
+    if (tol > 0.f && s->spent + est <= (double)tol) {
         s->spent += est;
         *used = 1;
         ++s->n_accepted;
-        e = hipMemcpyAsync(field, s->pred, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
+        e = forward(s, s->p0, field, ctx->rate[sub], false);
+        // in place: the interior from p0 and the head, the outer layers stay p1
+        if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, field, st);
     } else {
         ++s->n_fallback;
-        for (int k = 0; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
+        for (int k = 1; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
     }
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
 }
@@ -560,10 +619,32 @@ int gell_surrogate_gate_stats(const gell_surrogate* s, int64_t* checked, int64_t
     return GELL_OK;
 }
 
+// One layer after an upsampling, as inference runs it: y ((2S)^3 x cout) =
+// act(conv3(up2(x)) + b) (+ skip when not NULL) computed on the coarse S^3
+// grid of x. x, skip, y BF16 device tensors; w_master [cout][cin][27] and
+// b[cout] FP32 on the host. Allocates; synchronises.
+int gell_nn_upconv3(const uint16_t* x, const float* w_master, const float* b, const uint16_t* skip, uint16_t* y,
+                    int32_t cin, int32_t cout, int32_t S, int32_t relu) {
+    if (!x || !w_master || !b || !y || S < 4 || S % 4) return sfail(nullptr, GELL_E_ARG, "bad upconv arguments");
+    const int64_t nw = (int64_t)cout * cin * 27;
+    float *dm = nullptr, *db = nullptr;
+    uint16_t* w8 = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dm), sizeof(float) * nw);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&db), sizeof(float) * cout);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w8), 2 * (size_t)up_weight_count(cin, cout));
+    if (e == hipSuccess) e = hipMemcpy(dm, w_master, sizeof(float) * nw, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, sizeof(float) * cout, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = up_weights(dm, cin, cout, w8, nullptr);
+    if (e == hipSuccess) e = upconv3(x, w8, db, y, cin, cout, S, relu, nullptr, skip);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    for (void* p : {(void*)dm, (void*)db, (void*)w8})
+        if (p) (void)hipFree(p);
+    return e == hipSuccess ? GELL_OK : sfail(nullptr, GELL_E_HIP, std::string("upconv3: ") + hipGetErrorString(e));
+}
+
 // Single 3x3x3 layer on an S^3 grid, for checking the kernels one at a time.
-// x, y BF16 device tensors; w_master FP32 [cout][cin][27] and b FP32 host or
-// device per `params_on_host`. Allocates and frees its own packing buffers;
-// synchronises.
+// x, y BF16 device tensors; w_master FP32 [cout][cin][27] and b FP32 on the
+// host. Allocates and frees its own packing buffers; synchronises.
 int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
                   int32_t S, int32_t relu, int32_t input_grad) {
     if (!x || !w_master || !b || !y || S < 8 || S % 8) return sfail(nullptr, GELL_E_ARG, "bad conv arguments");

@@ -73,7 +73,6 @@ class SurrogateConfig(C.Structure):
     _fields_ = [
         ("N", C.c_int32),
         ("K", C.c_int32),
-        ("boundary", C.c_float),
         ("rate_scale", C.c_float),
         ("out_scale", C.c_float),
         ("check_every", C.c_int32),
@@ -155,14 +154,15 @@ def load_library() -> C.CDLL:
         "gell_surrogate_param_count": (i64, [vp]),
         "gell_surrogate_get_params": (C.c_int, [vp, vp, i64]),
         "gell_surrogate_set_params": (C.c_int, [vp, vp, i64]),
-        "gell_surrogate_predict": (C.c_int, [vp, vp, vp, vp]),
-        "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, i32, P(C.c_float), vp]),
+        "gell_surrogate_predict": (C.c_int, [vp, vp, vp, vp, vp]),
+        "gell_surrogate_gradients": (C.c_int, [vp, vp, vp, vp, vp, i32, P(C.c_float), vp]),
         "gell_surrogate_adam": (C.c_int, [vp, C.c_float]),
         "gell_surrogate_advance": (C.c_int, [vp, vp, i32, C.c_float, P(i32), P(C.c_float)]),
         "gell_surrogate_reset_budget": (C.c_int, [vp]),
         "gell_surrogate_gate_stats": (C.c_int, [vp, P(i64), P(i64), P(i64), P(C.c_double), P(C.c_double)]),
         "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
         "gell_nn_conv3_wgrad": (C.c_int, [vp, vp, vp, i32, i32, i32]),
+        "gell_nn_upconv3": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, i32]),
         "gell_io_last_error": (C.c_char_p, []),
         "gell_sphere_init": (C.c_int, [vp, i32, i32, P(Params), C.c_float, u64]),
         "gell_write_cells_csv": (C.c_int, [vp, i32, C.c_char_p]),
@@ -534,14 +534,18 @@ class Surrogate:
         a = np.ascontiguousarray(a, np.float32)
         self._check(self.lib.gell_surrogate_set_params(self.h, a.ctypes.data, a.size), "set_params")
 
-    def predict(self, field: int, rate: int, out: int) -> None:
-        self._check(self.lib.gell_surrogate_predict(self.h, _ptr(field), _ptr(rate), _ptr(out)), "predict")
+    def predict(self, field, field1, rate, out) -> None:
+        """out = the field K substeps after `field` under `rate`; field1 is
+        `field` after one real substep under it (device pointers)."""
+        self._check(self.lib.gell_surrogate_predict(self.h, _ptr(field), _ptr(field1), _ptr(rate), _ptr(out)),
+                    "predict")
 
-    def gradients(self, field, rate, target, want_grad: bool = False, accumulate: bool = False):
+    def gradients(self, field, field1, rate, target, want_grad: bool = False, accumulate: bool = False):
         loss = C.c_float()
         g = np.zeros(self.nparam, np.float32) if want_grad else None
-        self._check(self.lib.gell_surrogate_gradients(self.h, _ptr(field), _ptr(rate), _ptr(target), int(accumulate),
-                                                      C.byref(loss), g.ctypes.data if want_grad else None),
+        self._check(self.lib.gell_surrogate_gradients(self.h, _ptr(field), _ptr(field1), _ptr(rate), _ptr(target),
+                                                      int(accumulate), C.byref(loss),
+                                                      g.ctypes.data if want_grad else None),
                     "gradients")
         return (loss.value, g) if want_grad else loss.value
 
@@ -584,6 +588,17 @@ def nn_conv3(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, cin
                                       int(input_grad))
     if st != 0:
         raise GellError(f"gell_nn_conv3: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
+
+
+def nn_upconv3(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, cin: int, cout: int, S: int,
+               relu: bool = True, skip: DeviceBuffer | None = None) -> None:
+    """A layer after an upsampling, on x's S^3 grid (y is (2S)^3 x cout)."""
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    st = load_library().gell_nn_upconv3(_ptr(x), w.ctypes.data, b.ctypes.data, _ptr(skip) if skip else None, _ptr(y),
+                                        cin, cout, S, int(relu))
+    if st != 0:
+        raise GellError(f"gell_nn_upconv3: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
 
 
 def nn_conv3_wgrad(x: DeviceBuffer, dy: DeviceBuffer, cin: int, cout: int, S: int) -> np.ndarray:

@@ -1,15 +1,24 @@
 """Training and evaluation of the LOD surrogate (gell_surrogate_* of the C
-ABI) on fields produced by the library's own LOD kernels.
+ABI) on fields produced by the library's own kernels.
 
-Samples come from simulated spheroids: cells are placed, mechanics steps
-(cell list, forces, uptake map, 11 LOD substeps each) advance the oxygen
-field from its initial value, and at chosen steps the pair (field, uptake
-map) is stored with the field K substeps later under that map (the
-trajectory then continues from the stored field). Training minimises the
-squared interior error with Adam; evaluation reports the relative L2 error
-of the predicted field and of the predicted update on held-out samples.
+A sample is what gell_surrogate_advance sees in the loop: the field p0 right
+after a mechanics step's uptake map, the field p1 one real substep later,
+the uptake map, and the target, the field K substeps after p0 under that map.
+Samples come from simulated spheroids (seeded cells with a radial phase
+profile: a proliferating rim, a quiescent shell, a necrotic core) run through
+biology steps of the library (birth, 61 mechanics steps of cell list,
+forces, uptake map and 11 LOD substeps, death): a few from the first
+biology step, where the oxygen field still relaxes from its initial value,
+the rest from later, quasi-steady biology steps, among them the mechanics
+step right after births. The trajectory goes on from the stored p0.
 
-    python -m gell_amd.surrogate_train --N 128 --trajectories 24 --steps 3000 \
+The network predicts the K-substep increment in units of the first
+substep's increment, so every sample weighs the same in the loss whatever
+its size. Evaluation reports, on held-out samples, the relative L2 error of
+the predicted increment and of the predicted field, beside those of keeping
+the field (p0) and of one substep (p1).
+
+    python -m gell_amd.surrogate_train --N 256 --trajectories 32 --steps 4000 \\
         --out weights.npy
 """
 from __future__ import annotations
@@ -35,83 +44,109 @@ class Sample:
     def __init__(self, N: int):
         nb = N ** 3 * 4
         self.p0 = rt.DeviceBuffer(nb)
+        self.p1 = rt.DeviceBuffer(nb)
         self.rate = rt.DeviceBuffer(nb)
         self.target = rt.DeviceBuffer(nb)
+        self.tag = ""
 
     def free(self):
-        for b in (self.p0, self.rate, self.target):
+        for b in (self.p0, self.p1, self.rate, self.target):
             b.free()
 
 
-def trajectory_samples(N: int, n_cells: int, seed: int, K: int, sample_steps: list[int], dx: float = 10.0):
-    """Samples along one spheroid trajectory (mechanics steps from a uniform
-    field; a sample at each step in sample_steps)."""
+def trajectory_samples(N: int, n_cells: int, radius: float, seed: int, K: int, picks: set[tuple[int, int]],
+                       dx: float = 10.0) -> list[Sample]:
+    """Samples at (biology step, mechanics step) pairs of one spheroid's run."""
     lib = rt.load_library()
-    P = rt.make_params(N=N, dx=dx, capacity=n_cells + 64, seed=seed, substrates=[OXYGEN])
-    radius = min(1200.0 * (n_cells / 1e6) ** (1 / 3), 0.42 * N * dx)
+    P = rt.make_params(N=N, dx=dx, capacity=2 * n_cells + 64, seed=seed, substrates=[OXYGEN])
     cells = spheroid.make_cells(n_cells, N, dx, radius, seed, capacity=P.capacity)
+    c = N * dx / 2.0
+    spheroid.radial_profile(cells, n_cells, (c, c, c), radius, seed + 1)
     ctx = rt.Context(P)
     buf = rt.DeviceBuffer(P.capacity * rt.CELL_DTYPE.itemsize)
     buf.upload(cells)
+    mech = rt.float_loop_count(P.dt_bio, P.dt_mech)
     diff = rt.float_loop_count(P.dt_mech, P.dt_diff)
-    out = []
     nb = N ** 3 * 4
     field = ctx.field_ptr(0)
     rate = lib.gell_uptake_field(ctx.h, 0)
-    for m in range(max(sample_steps) + 1):
-        ctx.fm_update(buf, n_cells)
-        ctx.prelod(0, buf, n_cells)
-        if m in sample_steps:
-            s = Sample(N)
-            ctx.sync()
-            lib.gell_copy_d2d(s.p0.ptr, field, nb)
-            lib.gell_copy_d2d(s.rate.ptr, rate, nb)
-            for _ in range(K):
+    out = []
+    n = n_cells
+    last_bio = max(b for b, _ in picks)
+    for b in range(last_bio + 1):
+        n = ctx.birth(buf, n, b + 1)
+        for m in range(mech):
+            ctx.fm_update(buf, n)
+            ctx.prelod(0, buf, n)
+            if (b, m) in picks:
+                s = Sample(N)
+                s.tag = f"bio {b} mech {m}"
+                ctx.sync()
+                lib.gell_copy_d2d(s.p0.ptr, field, nb)
+                lib.gell_copy_d2d(s.rate.ptr, rate, nb)
                 ctx.lod_substep(0)
-            ctx.sync()
-            lib.gell_copy_d2d(s.target.ptr, field, nb)
-            lib.gell_copy_d2d(field, s.p0.ptr, nb)
-            out.append(s)
-        for _ in range(diff):
-            ctx.lod_substep(0)
+                ctx.sync()
+                lib.gell_copy_d2d(s.p1.ptr, field, nb)
+                for _ in range(K - 1):
+                    ctx.lod_substep(0)
+                ctx.sync()
+                lib.gell_copy_d2d(s.target.ptr, field, nb)
+                lib.gell_copy_d2d(field, s.p0.ptr, nb)
+                out.append(s)
+            for _ in range(diff):
+                ctx.lod_substep(0)
+        n = ctx.death(buf, n)
     ctx.sync()
     buf.free()
     ctx.close()
     return out
 
 
-def make_dataset(N: int, trajectories: int, K: int, seed: int, per_traj: int = 6, max_step: int = 240,
-                 early: int = 20):
-    """Half of each trajectory's samples come from its first `early`
-    mechanics steps, where the field still relaxes from its initial value and
-    the K-substep update is large; the rest from the quasi-steady remainder."""
+def make_dataset(N: int, trajectories: int, K: int, seed: int, per_traj: int = 6, bio_steps: int = 3):
+    """Spheroids of 100k..1M cells at 256^3 (scaled by (N/256)^3), radius
+    giving 0.1..0.3 cells per voxel; per trajectory a third of the samples in
+    the first biology step's first 20 mechanics steps, the rest in later
+    biology steps, one of them at the mechanics step right after births."""
     rng = np.random.default_rng(seed)
     data = []
     for t in range(trajectories):
-        n_cells = int(np.exp(rng.uniform(np.log(2e4), np.log(1.6e5))) * (N / 128) ** 3)
-        h = per_traj // 2
-        steps = sorted(set(int(x) for x in np.concatenate([rng.integers(0, early, h),
-                                                             rng.integers(early, max_step, per_traj - h)])))
-        data += trajectory_samples(N, n_cells, int(seed + 1000 * t), K, steps)
-        log(f"[train] trajectory {t}: {n_cells} cells, samples at mechanics steps {steps}")
+        n_cells = int(np.exp(rng.uniform(np.log(1e5), np.log(1e6))) * (N / 256) ** 3)
+        density = rng.uniform(0.1, 0.3)  # cells per voxel (10 um voxels)
+        radius = min((3 * n_cells / (4 * np.pi * density)) ** (1 / 3) * 10.0, 0.42 * N * 10.0)
+        early = per_traj // 3
+        picks = {(0, int(m)) for m in rng.choice(20, early, replace=False)}
+        picks.add((int(rng.integers(1, bio_steps)), 0))
+        while len(picks) < per_traj:
+            picks.add((int(rng.integers(1, bio_steps)), int(rng.integers(0, 61))))
+        t0 = time.perf_counter()
+        data += trajectory_samples(N, n_cells, radius, int(seed + 1000 * t), K, picks)
+        log(f"[train] trajectory {t}: {n_cells} cells, radius {radius:.0f} um, samples at {sorted(picks)} "
+            f"({time.perf_counter() - t0:.1f} s)")
     return data
 
 
-def sq_errors(sur: rt.Surrogate, s: Sample, N: int, out: rt.DeviceBuffer) -> tuple[float, float, float]:
-    """(|pred - target|^2, |target|^2, |target - p0|^2) of one sample."""
-    sur.predict(s.p0, s.rate, out)
-    pred = out.download(np.float32, N ** 3).astype(np.float64)
-    p0 = s.p0.download(np.float32, N ** 3).astype(np.float64)
-    tgt = s.target.download(np.float32, N ** 3).astype(np.float64)
-    return float(((pred - tgt) ** 2).sum()), float((tgt ** 2).sum()), float(((tgt - p0) ** 2).sum())
+def sq_errors(sur: rt.Surrogate, s: Sample, N: int, out: rt.DeviceBuffer) -> dict:
+    """Squared norms of one sample: prediction error, target, and the errors
+    of keeping p0 and of taking p1."""
+    sur.predict(s.p0, s.p1, s.rate, out)
+    V = N ** 3
+    pred = out.download(np.float32, V).astype(np.float64)
+    p0 = s.p0.download(np.float32, V).astype(np.float64)
+    p1 = s.p1.download(np.float32, V).astype(np.float64)
+    tgt = s.target.download(np.float32, V).astype(np.float64)
+    return {"pred": float(((pred - tgt) ** 2).sum()), "target": float((tgt ** 2).sum()),
+            "keep": float(((tgt - p0) ** 2).sum()), "one": float(((tgt - p1) ** 2).sum())}
 
 
-def summarize(errs) -> dict:
-    e = np.array(errs)
-    field = np.sqrt(e[:, 0] / e[:, 1])
+def summarize(errs: list[dict]) -> dict:
+    e = {k: np.array([d[k] for d in errs]) for k in errs[0]}
+    field = np.sqrt(e["pred"] / e["target"])
+    incr = np.sqrt(e["pred"] / np.maximum(e["keep"], 1e-300))
     return {"field_rel_l2_mean": float(field.mean()), "field_rel_l2_max": float(field.max()),
-            "update_rel_l2": float(np.sqrt(e[:, 0].sum() / max(e[:, 2].sum(), 1e-30))),
-            "zero_update_field_rel_l2_mean": float(np.sqrt(e[:, 2] / e[:, 1]).mean())}
+            "increment_rel_l2_mean": float(incr.mean()), "increment_rel_l2_max": float(incr.max()),
+            "keep_field_rel_l2_mean": float(np.sqrt(e["keep"] / e["target"]).mean()),
+            "one_substep_field_rel_l2_mean": float(np.sqrt(e["one"] / e["target"]).mean()),
+            "one_substep_increment_rel_l2_mean": float(np.sqrt(e["one"] / np.maximum(e["keep"], 1e-300)).mean())}
 
 
 def train(sur: rt.Surrogate, data: list[Sample], steps: int, lr: float, seed: int, batch: int = 4,
@@ -126,7 +161,7 @@ def train(sur: rt.Surrogate, data: list[Sample], steps: int, lr: float, seed: in
         loss = 0.0
         for b in range(batch):
             s = data[int(rng.integers(len(data)))]
-            loss += sur.gradients(s.p0, s.rate, s.target, accumulate=b > 0) / batch
+            loss += sur.gradients(s.p0, s.p1, s.rate, s.target, accumulate=b > 0) / batch
         sur.adam(float(lr_t))
         hist.append(loss)
         if (it + 1) % report_every == 0:
@@ -137,11 +172,11 @@ def train(sur: rt.Surrogate, data: list[Sample], steps: int, lr: float, seed: in
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--N", type=int, default=128)
+    ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--K", type=int, default=50)
-    ap.add_argument("--trajectories", type=int, default=24)
+    ap.add_argument("--trajectories", type=int, default=32)
     ap.add_argument("--per-traj", type=int, default=6)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
@@ -160,13 +195,17 @@ def main() -> int:
     out = rt.DeviceBuffer(args.N ** 3 * 4)
     before = summarize([sq_errors(sur, s, args.N, out) for s in held])
     hist, secs = train(sur, train_set, args.steps, args.lr, args.seed, args.batch)
-    after = summarize([sq_errors(sur, s, args.N, out) for s in held])
+    after_each = [sq_errors(sur, s, args.N, out) for s in held]
+    after = summarize(after_each)
     fit = summarize([sq_errors(sur, s, args.N, out) for s in train_set[:24]])
     np.save(args.out, sur.params())
     rep = {
         "N": args.N, "K": args.K, "train_samples": len(train_set), "held_out": len(held), "steps": args.steps,
         "train_seconds": secs, "final_loss": float(np.mean(hist[-100:])),
         "held_before_training": before, "held": after, "train_subset": fit,
+        "held_per_sample": [{"tag": s.tag, "increment_rel_l2": float(np.sqrt(d["pred"] / max(d["keep"], 1e-300))),
+                             "keep_field_rel_l2": float(np.sqrt(d["keep"] / d["target"]))}
+                            for s, d in zip(held, after_each)],
     }
     print(json.dumps(rep), flush=True)
     out.free()

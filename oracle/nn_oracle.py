@@ -3,7 +3,11 @@ nn.hip) in numpy float64 -- test infrastructure only (imported by tests/ as
 the checker; the product never imports it).
 
 The network has no counterpart in the reference: the reference's loop runs
-every LOD substep (Main.cu:121-123). What pins this restatement:
+every LOD substep (Main.cu:121-123). It predicts the field K substeps after
+p0 from the first substep's increment r = p1 - p0, given in units of its
+interior RMS s, and the uptake map; the prediction is p0 + s out_scale head
+in the interior and p1 on the two outer layers of every face. What pins this
+restatement:
   * its analytic backward against central finite differences of its own
     forward (float64, no rounding) -- tests/test_nn_oracle.py;
   * each conv layer against a direct per-voxel triple loop on tiny grids;
@@ -113,13 +117,28 @@ def up_back(dy, rnd=ident):
     return rnd(dy.reshape(S, 2, S, 2, S, 2, -1).sum(axis=(1, 3, 5)))
 
 
-def features(p, rate, bnd, rate_scale, rnd=ident):
+def increment_scale(p0, p1):
+    """s = RMS of p1 - p0 over the interior (voxels two or more from every
+    face), and 1 / s (0 when s = 0)."""
+    m = interior(p0.shape[0])
+    r = (np.asarray(p1, np.float64) - np.asarray(p0, np.float64))[m]
+    s = float(np.sqrt((r * r).mean()))
+    return s, (1.0 / s if s > 0 else 0.0)
+
+
+def features(p0, p1, rate, rate_scale, rnd=ident):
     """24-channel input on the half grid (fine arrays indexed [z, y, x]):
-    q = p / bnd - 1 at the 8 fine voxels of each coarse voxel as a rounded
+    q = (p1 - p0) / s at the 8 fine voxels of each coarse voxel as a rounded
     high part and the rounded remainder, then rate times rate_scale at the
     same 8 (fine voxel (z & 1) 4 + (y & 1) 2 + (x & 1)). Without rounding the
-    remainder channels are zero."""
-    q = fine_to_head(p / bnd - 1.0)
+    remainder channels are zero. With rounding (the device's emulation) q is
+    formed in float32, as the device forms it."""
+    _, inv = increment_scale(p0, p1)
+    if rnd is ident:
+        q = fine_to_head((np.asarray(p1, np.float64) - np.asarray(p0, np.float64)) * inv)
+    else:
+        d = (np.float32(p1) - np.float32(p0)) * np.float32(inv)
+        q = fine_to_head(d.astype(np.float64))
     hi = rnd(q)
     return np.concatenate([hi, rnd(q - hi), rnd(fine_to_head(rate * rate_scale))], axis=-1)
 
@@ -141,13 +160,14 @@ def interior(N):
     return m
 
 
-def forward(params, p, rate, cfg, rnd=ident, keep=False):
-    """Predicted field (N^3) and, with keep, the activations for backward."""
+def forward(params, p, p1, rate, cfg, rnd=ident, keep=False):
+    """Predicted field (N^3) K substeps after p, given p1 = p after one
+    substep; with keep, also the activations for backward."""
     Wb = unpack(params)
     wr = [(rnd(W), b) for W, b in Wb]
     c = lambda l, x: conv3(x, wr[l][0], wr[l][1], LAYERS[l][3], rnd)  # noqa: E731
     t = {}
-    t["f0"] = features(p, rate, cfg["boundary"], cfg["rate_scale"], rnd)
+    t["f0"] = features(p, p1, rate, cfg["rate_scale"], rnd)
     t["a0"] = c(0, t["f0"])
     t["e1"] = c(1, t["a0"])
     t["p1"] = pool(t["e1"], rnd)
@@ -172,26 +192,31 @@ def forward(params, p, rate, cfg, rnd=ident, keep=False):
     t["x1"] = rnd(t["c1"] + t["e1"])
     t["d1"] = c(13, t["x1"])
     t["h"] = c(14, t["d1"])
-    out = p + cfg["out_scale"] * head_to_fine(t["h"])
-    out = np.where(interior(p.shape[0]), out, p)
+    s, _ = increment_scale(p, p1)
+    out = p + s * cfg["out_scale"] * head_to_fine(t["h"])
+    out = np.where(interior(p.shape[0]), out, p1)
     return (out, t, wr) if keep else out
 
 
-def loss(params, p, rate, target, cfg, rnd=ident):
-    """Mean squared interior error in out_scale units (forward only)."""
-    out = forward(params, p, rate, cfg, rnd)
+def loss(params, p, p1, rate, target, cfg, rnd=ident):
+    """Mean squared interior error of the increment in units of s out_scale
+    (forward only)."""
+    out = forward(params, p, p1, rate, cfg, rnd)
     N = p.shape[0]
-    e = np.where(interior(N), (out - target) / cfg["out_scale"], 0.0)
+    _, inv = increment_scale(p, p1)
+    e = np.where(interior(N), (out - target) * inv / cfg["out_scale"], 0.0)
     return float((e * e).sum() / float((N - 4) ** 3))
 
 
-def loss_and_grad(params, p, rate, target, cfg, rnd=ident):
-    """Mean squared interior error in out_scale units, and d loss / d params."""
-    out, t, wr = forward(params, p, rate, cfg, rnd, keep=True)
+def loss_and_grad(params, p, p1, rate, target, cfg, rnd=ident):
+    """Mean squared interior error of the increment in units of s out_scale,
+    and d loss / d params."""
+    out, t, wr = forward(params, p, p1, rate, cfg, rnd, keep=True)
     N = p.shape[0]
     m = interior(N)
     cnt = float((N - 4) ** 3)
-    e = np.where(m, head_to_fine(t["h"]) - (target - p) / cfg["out_scale"], 0.0)
+    _, inv = increment_scale(p, p1)
+    e = np.where(m, head_to_fine(t["h"]) - (target - p) * inv / cfg["out_scale"], 0.0)
     loss = float((e * e).sum() / cnt)
     dh = rnd(fine_to_head(2.0 * e / cnt))
     grads = [None] * len(LAYERS)

@@ -41,6 +41,36 @@ def close_bf16(got, want):
     return np.all(err <= 2.0 ** -7 * np.abs(want) + 2.0 ** -9 * 1e-2 * scale), float(err.max() / scale)
 
 
+@pytest.mark.parametrize("cin,cout,S", [(128, 64, 8), (64, 32, 8), (32, 16, 16)])
+def test_upconv_layer_equals_the_layer_on_the_upsampled_tensor(cin, cout, S):
+    """The layers after an upsampling run on the coarse grid as 8 per-parity
+    2x2x2 convolutions with summed FP32 taps (rounded to BF16 once); against
+    the oracle's 3x3x3 convolution of the formed (2S)^3 upsampling with the
+    unrounded weights, ReLU, BF16, + skip, BF16: within the rounding of the
+    combined weights (2^-9 relative each) and of the outputs."""
+    rng = np.random.default_rng(cin * 7 + cout)
+    xb = to_bf16_bits(rng.normal(size=(S, S, S, cin)))
+    x = from_bits(xb)
+    W = rng.normal(0, np.sqrt(2 / (27 * cin)), size=(cout, cin, 27)).astype(np.float32).astype(np.float64)
+    b = rng.normal(0, 0.1, size=cout).astype(np.float32).astype(np.float64)
+    F = 2 * S
+    skb = to_bf16_bits(rng.normal(size=(F, F, F, cout)))
+    xd, skd = dev(xb), dev(skb)
+    yd = rt.DeviceBuffer(F ** 3 * cout * 2)
+    conv = nno.conv3(nno.up(x), W, b, True)
+    for with_skip in (False, True):
+        rt.nn_upconv3(xd, W, b, yd, cin, cout, S, relu=True, skip=skd if with_skip else None)
+        got = from_bits(yd.download(np.uint16, F ** 3 * cout)).reshape(F, F, F, cout)
+        want = nno.bf16(conv)
+        if with_skip:
+            want = nno.bf16(want + from_bits(skb).reshape(F, F, F, cout))
+        e = rel_l2(got, want)
+        log(f"[nn] upconv {cin}->{cout} at {S}^3 -> {F}^3, skip {with_skip}: rel L2 {e:.2e}")
+        assert e < 4e-3
+    for buf in (xd, skd, yd):
+        buf.free()
+
+
 @pytest.mark.parametrize("cin,cout", SHAPES)
 def test_conv_layer_forward_and_gradients(cin, cout):
     S = 16 if cin <= 64 else 8
@@ -73,7 +103,7 @@ def test_conv_layer_forward_and_gradients(cin, cout):
         buf.free()
 
 
-CFG = dict(boundary=38.0, rate_scale=0.1, out_scale=1.0)
+CFG = dict(rate_scale=0.1, out_scale=8.0)
 
 
 @pytest.fixture(scope="module")
@@ -81,8 +111,9 @@ def net_problem():
     N = 128
     rng = np.random.default_rng(11)
     p = (38.0 - 6.0 * rng.random((N, N, N))).astype(np.float32)
+    p1 = (p - 0.05 * rng.random((N, N, N))).astype(np.float32)
     rate = (-25.0 * (rng.random((N, N, N)) < 0.1)).astype(np.float32)
-    target = (p - 2.0 * rng.random((N, N, N))).astype(np.float32)
+    target = (p + 8.0 * (p1 - p) * (1.0 + rng.random((N, N, N)))).astype(np.float32)
     sur = rt.Surrogate(N)
     params = sur.params().astype(np.float64)
     # non-zero biases so the bias paths carry signal
@@ -91,30 +122,38 @@ def net_problem():
         params[b_off[l]:b_off[l] + cout] = 0.05 * rng.normal(size=cout)
     params = params.astype(np.float32)
     sur.set_params(params)
-    bufs = [dev(a) for a in (p, rate, target)]
-    yield N, p, rate, target, params.astype(np.float64), sur, bufs
+    bufs = [dev(a) for a in (p, p1, rate, target)]
+    yield N, p, p1, rate, target, params.astype(np.float64), sur, bufs
     for b in bufs:
         b.free()
     sur.close()
 
 
 def test_network_prediction_matches_oracle(net_problem):
-    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    N, p, p1, rate, target, params, sur, (pd, p1d, rd, td) = net_problem
     out = rt.DeviceBuffer(N ** 3 * 4)
-    sur.predict(pd, rd, out)
+    sur.predict(pd, p1d, rd, out)
     got = out.download(np.float32, N ** 3).reshape(N, N, N).astype(np.float64)
-    want = nno.forward(params, p.astype(np.float64), rate.astype(np.float64), CFG, rnd=nno.bf16)
+    want = nno.forward(params, p.astype(np.float64), p1.astype(np.float64), rate.astype(np.float64), CFG,
+                       rnd=nno.bf16)
     e = rel_l2(got - p, want - p)
     log(f"[nn] prediction rel L2 of the update vs oracle: {e:.3e}")
     assert e < 1e-2
+    m = nno.interior(N)
+    assert np.array_equal(got[~m], p1.astype(np.float64)[~m])  # outer layers: the substep's
+    # out may alias field1: the prediction written over it is the same
+    p1copy = dev(p1)
+    sur.predict(pd, p1copy, rd, p1copy)
+    assert np.array_equal(p1copy.download(np.float32, N ** 3), out.download(np.float32, N ** 3))
+    p1copy.free()
     out.free()
 
 
 def test_network_gradients_match_oracle(net_problem):
-    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
-    loss, g = sur.gradients(pd, rd, td, want_grad=True)
-    wl, wg = nno.loss_and_grad(params, p.astype(np.float64), rate.astype(np.float64), target.astype(np.float64), CFG,
-                               rnd=nno.bf16)
+    N, p, p1, rate, target, params, sur, (pd, p1d, rd, td) = net_problem
+    loss, g = sur.gradients(pd, p1d, rd, td, want_grad=True)
+    wl, wg = nno.loss_and_grad(params, p.astype(np.float64), p1.astype(np.float64), rate.astype(np.float64),
+                               target.astype(np.float64), CFG, rnd=nno.bf16)
     log(f"[nn] loss device {loss:.6f} oracle {wl:.6f}")
     assert abs(loss - wl) <= 1e-2 * wl
     w_off, b_off, _ = nno.param_layout()
@@ -131,10 +170,10 @@ def test_network_gradients_match_oracle(net_problem):
 def test_adam_step_matches_its_formula(net_problem):
     """One device Adam step from fresh moments equals the host formula on the
     returned gradients: w - lr (g (1 - b1) / c1) / (sqrt(g^2 (1 - b2) / c2) + eps)."""
-    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    N, p, p1, rate, target, params, sur, (pd, p1d, rd, td) = net_problem
     sur.set_params(params.astype(np.float32))  # also clears the moments
     w0 = sur.params().astype(np.float64)
-    _, g = sur.gradients(pd, rd, td, want_grad=True)
+    _, g = sur.gradients(pd, p1d, rd, td, want_grad=True)
     lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
     sur.adam(lr)
     w1 = sur.params().astype(np.float64)
@@ -145,13 +184,14 @@ def test_adam_step_matches_its_formula(net_problem):
 
 
 def test_training_reduces_the_loss(net_problem):
-    """A uniform shift of the interior: 40 small Adam steps lower the loss."""
-    N, p, rate, target, params, sur, (pd, rd, td) = net_problem
+    """A uniform multiple of the first increment: 40 small Adam steps lower
+    the loss."""
+    N, p, p1, rate, target, params, sur, (pd, p1d, rd, td) = net_problem
     sur.set_params(params.astype(np.float32))
-    shifted = dev((p - 0.1).astype(np.float32))
+    shifted = dev((p + 20.0 * (p1 - p)).astype(np.float32))
     losses = []
     for _ in range(40):
-        losses.append(sur.gradients(pd, rd, shifted))
+        losses.append(sur.gradients(pd, p1d, rd, shifted))
         sur.adam(3e-4)
     log(f"[nn] losses {losses[0]:.5f} -> {min(losses[-5:]):.5f}")
     assert min(losses[-5:]) < 0.7 * losses[0]
