@@ -497,7 +497,7 @@ int gell_tune(gell_ctx* ctx, int knob, int value) {
     if (knob < 0 || knob >= GELL_N_KNOBS) return fail(ctx, GELL_E_ARG, "unknown knob");
     const bool parts = knob == GELL_KNOB_LOD_PARTS || knob == GELL_KNOB_LOD_X_PARTS;
     const bool ok = parts ? (value == 1 || value == 2 || value == 4 || value == 8 || value == 16)
-                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 9));
+                          : (value == 0 || value == 1 || (knob == GELL_KNOB_FM_PREFETCH && value >= 2 && value <= 10));
     if (!ok) return fail(ctx, GELL_E_ARG, "knob value out of range");
     const hipError_t e = gell::join_substrates(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "tune");

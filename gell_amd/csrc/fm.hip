@@ -1355,13 +1355,19 @@ hipError_t fm_ranges_linear(gell_ctx* ctx, int2* out) {
     return hipGetLastError();
 }
 
-// The variant a launch for n cells runs: the rank-compacted list (9) costs
-// four small extra launches per step, which below kRankMin cells outweigh
-// what its walk saves, so there variant 7 runs instead (same bits).
-constexpr int32_t kRankMin = 131072;
+// The variant a launch for n cells runs. 0-9 run as set; 10 (the default)
+// picks the rank-compacted list (9) for large sparse tissue and 7 otherwise
+// (same bits). The list costs four extra launches per step and pays only
+// where most stencil voxels are empty and there are many cells: measured,
+// 1M cells in 256^3 (1/16 cell per grid voxel) 92.0 -> 90.1 ms per biology
+// step with it; 10k cells in 64^3, 131k-200k-cell replicates in 128^3-192^3
+// and the reference's dense 100^3 example (up to ~1 cell per voxel) slower.
+constexpr int32_t kRankMin = 524288;
 int fm_variant(const gell_ctx* ctx, int32_t n) {
     const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
-    return (v == 9 && grid_count(ctx, n) < kRankMin) ? 7 : v;
+    if (v != 10) return v;
+    const int64_t m = grid_count(ctx, n);
+    return (m >= kRankMin && 8 * m <= (int64_t)ctx->N * ctx->N * ctx->N) ? 9 : 7;
 }
 
 hipError_t fm_keys_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
@@ -1432,7 +1438,7 @@ hipError_t fm_gather_launch(gell_ctx* ctx, const gell_cell* cells, int32_t n) {
 
 bool fm_fuses_uptake(const gell_ctx* ctx) {
     const int v = ctx->tune[GELL_KNOB_FM_PREFETCH];
-    return v >= 7 && v <= 9;
+    return v >= 7 && v <= 10;
 }
 
 hipError_t fm_force_move_launch(gell_ctx* ctx, gell_cell* cells, int32_t n, int32_t* uptake_list) {

@@ -109,7 +109,7 @@ struct gell_ctx {
     int64_t blk_cap = 0;
 
     // Kernel-variant knobs (GELL_KNOB_*).
-    int tune[GELL_N_KNOBS] = {1, 9, 4, 4, 1, 1};
+    int tune[GELL_N_KNOBS] = {1, 10, 4, 4, 1, 1};
 
     // Every substrate runs its LOD substeps on a stream of its own when
     // GELL_KNOB_SUBSTRATE_STREAMS is set: forked from the context stream after

@@ -233,13 +233,14 @@ int gell_sync(gell_ctx* ctx);
  *                              bits and the rank of its first occupied voxel
  *                              (16 bytes), and per occupied voxel its first
  *                              live slot (4 bytes), instead of the 8-byte
- *                              Morton-indexed range of every voxel; for
- *                              fewer than 131072 cells (capacity, with
- *                              device-resident counts) 7 runs instead, the
- *                              list's extra launches costing more there.
- *                              Variants 7-9 also accumulate the uptake maps
+ *                              Morton-indexed range of every voxel,
+ *                          10 = 9 for at least 524288 cells (capacity, with
+ *                              device-resident counts) and at most one cell
+ *                              per 8 grid voxels, 7 otherwise: the list's
+ *                              extra launches pay only on large sparse
+ *                              tissue. Default 10.
+ *                              Variants 7-10 also accumulate the uptake maps
  *                              in gell_mech_step.
- *                              Default 9.
  *   GELL_KNOB_LOD_PARTS    y/z sweeps: 1 = one lane solves a whole line (the
  *                              reference's serial recurrence, bit-identical
  *                              to it), 2, 4, 8 or 16 = each line split into
@@ -417,8 +418,12 @@ int gell_ensemble_stats(const gell_ensemble* ens, int64_t* captures, int64_t* la
  * LOD surrogate (no counterpart in the reference, whose loop runs every
  * substep, Main.cu:121-123): a four-level 3-D U-Net (16..128 channels, BF16
  * matrix-core convolutions with FP32 accumulation) on the (N/2)^3 grid that
- * maps (field, uptake map) to the field after K substeps with that map held
- * fixed. Forward, gradients and Adam run on the device. In the loop
+ * predicts the field after K substeps with the uptake map held fixed. With
+ * the map fixed a substep is affine in the field, so the K-substep increment
+ * is a linear map, depending only on the uptake map, of the first substep's
+ * increment r = field1 - field: the network takes r in units of its interior
+ * RMS s (plus the uptake map) and returns the K-substep increment in units of
+ * s out_scale. Forward, gradients and Adam run on the device. In the loop
  * (gell_surrogate_advance) plain LOD stays the fallback and the judge: an
  * error budget decides whether a prediction stands in for the K substeps,
  * calibrated against the real K substeps every few calls.
@@ -427,9 +432,8 @@ typedef struct gell_surrogate gell_surrogate;
 typedef struct {
     int32_t N;        /* fine grid (multiple of 128) */
     int32_t K;        /* substeps one prediction stands for */
-    float boundary;   /* boundary value (input scaling; outer layers) */
-    float rate_scale; /* input scaling of the uptake map */
-    float out_scale;  /* field units of one head unit */
+    float rate_scale; /* input scaling of the uptake map (0.1) */
+    float out_scale;  /* units of s in one head unit (8) */
     int32_t check_every; /* gell_surrogate_advance: one call in this many runs
                             the K real substeps to calibrate the gate (6) */
     uint64_t seed;    /* weight initialisation */
@@ -443,32 +447,38 @@ const char* gell_surrogate_last_error(const gell_surrogate* sur);
 int64_t gell_surrogate_param_count(const gell_surrogate* sur);
 int gell_surrogate_get_params(gell_surrogate* sur, float* host, int64_t n);
 int gell_surrogate_set_params(gell_surrogate* sur, const float* host, int64_t n);
-/* out = predicted field after K substeps (device pointers to N^3 floats). */
-int gell_surrogate_predict(gell_surrogate* sur, const float* field, const float* rate, float* out);
-/* Loss (mean squared interior error in out_scale units) and parameter
+/* out = predicted field after K substeps from `field` under the uptake map
+ * `rate`; field1 = `field` after one real substep under it (device pointers
+ * to N^3 floats; out may alias field1). The two outer layers of every face,
+ * which each substep first resets to the boundary value, are field1's. */
+int gell_surrogate_predict(gell_surrogate* sur, const float* field, const float* field1, const float* rate,
+                           float* out);
+/* Loss (mean squared interior error of the increment target - field, in
+ * units of s out_scale, s the interior RMS of field1 - field) and parameter
  * gradients for one sample, replacing the stored gradients or, with
  * accumulate, added to them (mini-batches); the stored gradients are copied
  * to grad_host when not NULL (param_count floats). Synchronises. */
-int gell_surrogate_gradients(gell_surrogate* sur, const float* field, const float* rate, const float* target,
-                             int32_t accumulate, float* loss, float* grad_host);
+int gell_surrogate_gradients(gell_surrogate* sur, const float* field, const float* field1, const float* rate,
+                             const float* target, int32_t accumulate, float* loss, float* grad_host);
 /* One Adam step (0.9, 0.999, 1e-8) on the last gradients. */
 int gell_surrogate_adam(gell_surrogate* sur, float lr);
-/* Advance substrate `s` of ctx by K substeps. Each call predicts and
- * measures the prediction's residual r, its relative L2 change under one
- * real substep (*residual). The first call and every check_every-th call
- * after run the K real substeps too and keep their exact result; the
- * prediction's true relative L2 error e against them calibrates the ratio
- * e / r (the largest seen). Between those checks a prediction is kept
- * (*used = 1) only while the estimated errors (1.5 ratio r) of the
- * predictions kept in the current window add up to at most `tol`, relative
- * L2; otherwise the K LOD substeps run (*used = 0). tol = 0 makes every
- * call exact (bit for bit K substeps). ctx must have grid N and run on the
- * surrogate's stream. Synchronises. */
+/* Advance substrate `s` of ctx by K substeps. Each call runs the first
+ * substep for real and measures the relative size of its increment,
+ * rho = ||field1 - field|| (interior) / ||field|| (*residual). The first call
+ * and every check_every-th call after also predict and then run the other
+ * K - 1 real substeps, keeping their exact result; the prediction's true
+ * relative L2 error e against them calibrates the ratio e / rho (the largest
+ * seen). Between those checks a prediction is made and kept (*used = 1) only
+ * while the estimated errors (1.5 ratio rho) of the predictions kept in the
+ * current window add up to at most `tol`, relative L2; otherwise the other
+ * K - 1 substeps run (*used = 0) and the network does not. tol = 0 makes
+ * every call exact (bit for bit K substeps). ctx must have grid N and run on
+ * the surrogate's stream. Synchronises. */
 int gell_surrogate_advance(gell_surrogate* sur, gell_ctx* ctx, int32_t s, float tol, int32_t* used, float* residual);
 /* Start a new error-budget window (e.g. at each biology step). */
 int gell_surrogate_reset_budget(gell_surrogate* sur);
 /* Calls of gell_surrogate_advance so far by outcome (exact check, prediction
- * kept, fallback to the K substeps), the calibrated error / residual ratio
+ * kept, fallback to the K substeps), the calibrated error / rho ratio
  * (< 0 before the first check) and the error spent in the window. */
 int gell_surrogate_gate_stats(const gell_surrogate* sur, int64_t* checked, int64_t* accepted, int64_t* fallback,
                               double* ratio, double* spent);
@@ -480,6 +490,13 @@ int gell_surrogate_gate_stats(const gell_surrogate* sur, int64_t* checked, int64
  * reverses. Allocates; synchronises. */
 int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
                   int32_t S, int32_t relu, int32_t input_grad);
+/* A layer that follows an upsampling (network layers 8, 10, 12: (cin, cout)
+ * = (128, 64), (64, 32), (32, 16)) as inference runs it: y ((2S)^3 x cout) =
+ * act(conv3x3x3(up2(x)) + b) (+ skip when not NULL) computed on x's S^3 grid
+ * (S a multiple of 4) as 8 per-parity 2x2x2 convolutions. x, skip, y device
+ * BF16 tensors; w_master, b on the host. Allocates; synchronises. */
+int gell_nn_upconv3(const uint16_t* x, const float* w_master, const float* b, const uint16_t* skip, uint16_t* y,
+                    int32_t cin, int32_t cout, int32_t S, int32_t relu);
 /* The layer's weight gradient into host dw_master ([cout][cin][27]). */
 int gell_nn_conv3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw_master, int32_t cin, int32_t cout,
                         int32_t S);

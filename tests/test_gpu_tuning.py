@@ -28,13 +28,14 @@ def test_lod_variants_identical(N):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000), (13, 3000), (64, 140001)])
+@pytest.mark.parametrize("N,n", [(64, 20000), (64, 20001), (100, 30000), (13, 3000), (64, 140001), (192, 530001)])
 def test_fm_variants_identical(N, n):
     """All force-kernel variants give the same bits; an odd count exercises
     the staged variant's 8-byte record tail, N = 100 Morton blocks that
     stick out of the grid (block variant), N = 13 occupancy words (aligned
     4^3 blocks) clipped by the grid's faces, and the mixed cells include dead
-    and outside ones."""
+    and outside ones. Variant 10 (auto) runs 7 in all but the last case,
+    where (>= 524288 cells, <= 1/8 cell per grid voxel) it runs 9."""
     P = params(N, capacity=n)
     cells = mixed_cells(N, 10.0, n, n, seed=4, dense=True)
     _all_fm_variants_equal(P, cells, n)
@@ -42,7 +43,7 @@ def test_fm_variants_identical(N, n):
 
 def _all_fm_variants_equal(P, cells, n, steps=3):
     outs = []
-    for v in range(10):
+    for v in range(11):
         st = GpuState(P, cells)
         st.ctx.tune("fm_prefetch", v)
         for _ in range(steps):
@@ -75,7 +76,7 @@ def test_bad_knob_is_an_error():
     with pytest.raises(rt.GellError):
         st.ctx.tune("lod_yz_lds", 7)
     with pytest.raises(rt.GellError):
-        st.ctx.tune("fm_prefetch", 10)
+        st.ctx.tune("fm_prefetch", 11)
     with pytest.raises(KeyError):
         st.ctx.tune("nope", 1)
     st.close()
