@@ -6,28 +6,6 @@ movement), LOD diffusion (x/y/z Thomas sweeps) and proliferation/death
 behind the C ABI of ``include/gell_abi.h``; ``gell_amd.runtime`` binds that
 ABI from Python.
 """
-from .runtime import (  # noqa: F401
-    CELL_DTYPE,
-    Context,
-    DeviceBuffer,
-    Ensemble,
-    GellError,
-    Params,
-    default_params,
-    float_loop_count,
-    load_library,
-    make_params,
-)
-
-__all__ = [
-    "CELL_DTYPE",
-    "Context",
-    "DeviceBuffer",
-    "Ensemble",
-    "GellError",
-    "Params",
-    "default_params",
-    "float_loop_count",
-    "load_library",
-    "make_params",
-]
+from .runtime import (  # noqa: F401 (re-exported)
+    CELL_DTYPE, Context, DeviceBuffer, Ensemble, GellError, Params, Surrogate, default_params, float_loop_count,
+    load_library, make_params)
