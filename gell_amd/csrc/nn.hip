@@ -38,7 +38,56 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
+typedef __attribute__((ext_vector_type(2))) float f2v;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf2v;
+
+// Two floats -> packed BF16 pair (low half = a), round to nearest even: one
+// v_cvt_pk_bf16_f32, the rounding f2bf spells out.
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{a, b}, bf2v));
+}
+
 constexpr int kThreads = 256;  // 4 waves
+
+// Stages a halo of HX x HY x HZ voxels x CH 16-byte channel chunks into LDS,
+// chunk (hz, hy, hx, ch) at lds[((hz HY + hy) HX + hx) CH + ch], straight from
+// memory without passing through registers. The halo's origin is voxel
+// (x0, y0, z0) of the X x Y x Z tensor `base` (CIN channels, BF16); voxels
+// outside it read zero (an offset past the buffer's end reads zero). A
+// thread's (hy, hx, ch) is worked out once per plane slot; each z plane then
+// costs it one bounds test and one address. All loads are in flight together;
+// stage_wait() waits for them (before the barrier).
+template <int HX, int HY, int HZ, int CH>
+__device__ __forceinline__ void stage_halo(const uint16_t* base, int X, int Y, int Z, int x0, int y0, int z0,
+                                           uint4* lds) {
+    constexpr int PC = HX * HY * CH;  // chunks per z plane
+    constexpr int CIN = CH * 8;
+    const uint32_t num_bytes = (uint32_t)((int64_t)X * Y * Z * CIN * 2);
+    const uint32_t plane = (uint32_t)X * (uint32_t)Y * CIN * 2u;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(base), (short)0, (int)num_bytes, 0x00020000);
+    const int wbase = (int)(threadIdx.x & ~63u);
+#pragma unroll
+    for (int j = 0; j < (PC + kThreads - 1) / kThreads; ++j) {
+        const int p = j * kThreads + (int)threadIdx.x;
+        if (p < PC) {
+            const int ch = p % CH, hv = p / CH;
+            const int gx = x0 + hv % HX, gy = y0 + hv / HX;
+            const bool okxy = gx >= 0 && gx < X && gy >= 0 && gy < Y;
+            const uint32_t row = (((uint32_t)gy * (uint32_t)X + (uint32_t)gx) * CIN + (uint32_t)ch * 8u) * 2u;
+#pragma unroll
+            for (int hz = 0; hz < HZ; ++hz) {
+                const int gz = z0 + hz;
+                const uint32_t vo = (okxy && gz >= 0 && gz < Z) ? row + (uint32_t)gz * plane : num_bytes;
+                // lane L of the wave lands at lds[hz PC + j kThreads + wbase + L]
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void*)(lds + hz * PC + j * kThreads + wbase), 16, vo, 0, 0,
+                    0);
+            }
+        }
+    }
+}
+__device__ __forceinline__ void stage_wait() { __builtin_amdgcn_s_waitcnt(0); }
 constexpr int kTX = 8, kTY = 8;
 
 // ---------------------------------------------------------------------------
@@ -69,15 +118,8 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
     const int zb = Z / TZ;
     const int bz = blockIdx.z % zb, ct0 = (blockIdx.z / zb) * CG;
     const int x0 = blockIdx.x * kTX - 1, y0 = blockIdx.y * kTY - 1, z0 = bz * TZ - 1;
-    for (int q = threadIdx.x; q < HV * CH; q += kThreads) {
-        const int ch = q % CH, hv = q / CH;
-        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-        const int gx = x0 + hx, gy = y0 + hy, gz = z0 + hz;
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (gx >= 0 && gx < X && gy >= 0 && gy < Y && gz >= 0 && gz < Z)
-            val = *reinterpret_cast<const uint4*>(x + (((int64_t)gz * Y + gy) * X + gx) * CIN + ch * 8);
-        halo[q] = val;
-    }
+    stage_halo<HX, HY, HZ, CH>(x, X, Y, Z, x0, y0, z0, halo);
+    stage_wait();
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
@@ -86,14 +128,20 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
     for (int a = 0; a < VPW; ++a)
 #pragma unroll
         for (int c = 0; c < CG; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
-    // in-block position of the voxel this lane feeds as a B column
-    int hbase[VPW];
-#pragma unroll
-    for (int a = 0; a < VPW; ++a) {
-        const int v = (wv + 4 * a) * 16 + r;
-        const int lx = v % kTX, ly = (v / kTX) % kTY, lz = v / (kTX * kTY);
-        hbase[a] = (lz * HY + ly) * HX + lx;
-    }
+    // In-block position of the voxel this lane feeds as a B column: tile a of
+    // wave wv is the 16 voxels (wv + 4 a) 16 .. + 15, i.e. z slice a, rows
+    // 2 wv and 2 wv + 1; so fragment a sits PLANE chunks after fragment 0 and
+    // every LDS read below is this lane's base plus a compile-time offset.
+    constexpr int PLANE = HX * HY * CH;
+    const int hbase0 = (2 * wv + (r >> 3)) * HX + (r & 7);
+    // B chunk of fragment a at step s: lane base (its voxel, and its k group's
+    // channel chunk and tap shift) + tap offset of the step + a PLANE.
+    // CIN >= 32: the four k groups of a step are four channel chunks of one
+    // tap. CIN = 16: groups 0, 1 are the channel halves of tap 2 s, groups 2,
+    // 3 those of tap 2 s + 1, whose offset from tap 2 s is one of three.
+    const uint4* lb1 = halo + hbase0 * CH + (CIN >= 32 ? g : (g & 1) + (g >> 1) * CH);
+    const uint4* lb2 = halo + hbase0 * CH + (g & 1) + (g >> 1) * CH * (HX - 2);
+    const uint4* lb3 = halo + hbase0 * CH + (g & 1) + (g >> 1) * CH * (HX * HY - 2 * HX - 2);
     const uint16_t* wr = w + ((ct0 * 16) + r) * KP + g * 8;
     // weight fragments of step s + 1 are loaded while step s computes
     bf8 af[CG], an[CG];
@@ -108,17 +156,31 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
 #pragma unroll
             for (int c = 0; c < CG; ++c) an[c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP + (s + 1) * 32);
         }
-        const bool valid = tap < 27;
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        const int off = (dz * HY + dy) * HX + dx;
+        // all of the step's B fragments are read before the first product,
+        // so the LDS reads overlap one another and the matrix-core work
+        bf8 bf[VPW];
+        if constexpr (CIN >= 32 || CIN == 16) {
+            const int kb = s * 32, t0 = kb / CIN, cb = kb % CIN;  // tap and chunk of k group 0
+            const int toff = (((t0 / 9) * HY + (t0 / 3) % 3) * HX + t0 % 3) * CH + cb / 8;
+            const uint4* p = CIN >= 32 || t0 % 3 != 2 ? lb1 : ((t0 / 3) % 3 != 2 ? lb2 : lb3);
+            const bool valid = CIN >= 32 || t0 + 1 < 27 || g < 2;  // the padding tap 27 reads zero
 #pragma unroll
-        for (int a = 0; a < VPW; ++a) {
-            uint4 raw = make_uint4(0, 0, 0, 0);
-            if (valid) raw = halo[(hbase[a] + off) * CH + c0 / 8];
-            const bf8 bf = __builtin_bit_cast(bf8, raw);
+            for (int a = 0; a < VPW; ++a)
+                bf[a] = __builtin_bit_cast(bf8, valid ? p[toff + a * PLANE] : make_uint4(0, 0, 0, 0));
+        } else {
+            const bool valid = tap < 27;
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+            const int off = (dz * HY + dy) * HX + dx;
 #pragma unroll
-            for (int c = 0; c < CG; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf, acc[a][c], 0, 0, 0);
+            for (int a = 0; a < VPW; ++a)
+                bf[a] = __builtin_bit_cast(
+                    bf8, valid ? halo[(hbase0 + a * HX * HY + off) * CH + c0 / 8] : make_uint4(0, 0, 0, 0));
         }
+#pragma unroll
+        for (int a = 0; a < VPW; ++a)
+#pragma unroll
+            for (int c = 0; c < CG; ++c)
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf[a], acc[a][c], 0, 0, 0);
     }
 #pragma unroll
     for (int a = 0; a < VPW; ++a) {
@@ -130,18 +192,122 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
         for (int c = 0; c < CG; ++c) {
             const int o = (ct0 + c) * 16 + 4 * g;
             if (o >= COUT) continue;
-            uint16_t h[4];
+            float t[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float t = acc[a][c][i] + bias[o + i];
-                if (relu && t < 0.f) t = 0.f;
-                h[i] = f2bf(t);
+                t[i] = acc[a][c][i] + bias[o + i];
+                if (relu) t[i] = fmaxf(t[i], 0.f);
             }
-            uint2 pk;
-            pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-            pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-            *reinterpret_cast<uint2*>(y + vox * COUT + o) = pk;
+            *reinterpret_cast<uint2*>(y + vox * COUT + o) = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward convolution for the coarse levels (CIN a multiple of 32), split
+// along K. Block = one 8 x 8 voxel z slice (4 tiles of 16 voxels: rows 2t,
+// 2t + 1) and CG 16-channel output tiles; its halo, 10 x 10 x 3 voxels x CIN,
+// sits in LDS. Wave w computes every tile of the block over its own quarter
+// of the K steps, so a wave issues 4 CG products per weight load (the plain
+// kernel, one wave per voxel tile, issued CG and waited on each weight load
+// in turn: the coarse levels have few blocks and long K loops, and that
+// latency bounded them). The weights come from memory in groups of U steps,
+// the next group in flight while the current one computes. The four
+// partial sums meet in LDS (over the halo, after a barrier); wave w adds up
+// tile w and applies the epilogue of conv3_fwd.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int CG>
+__global__ __launch_bounds__(kThreads) void conv3_fwd_splitk(const uint16_t* __restrict__ x,
+                                                             const uint16_t* __restrict__ w,
+                                                             const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                             int X, int Y, int Z, int relu) {
+    constexpr int HX = kTX + 2, HY = kTY + 2, HZ = 3;
+    constexpr int CH = CIN / 8;
+    constexpr int KP = 27 * CIN;  // a multiple of 32: no padding
+    constexpr int KSTEPS = KP / 32;
+    constexpr int KW = (KSTEPS + 3) / 4;  // K steps per wave
+    constexpr int U = 4;                  // weight steps per load group
+    constexpr int NCT = cpad(COUT) / 16;
+    constexpr int HALO = HX * HY * HZ * CH;       // uint4 chunks
+    constexpr int RED = 4 * 4 * CG * 64;          // 4 waves x 4 tiles x CG x 64 lanes (float4 each)
+    static_assert(CIN % 32 == 0 && NCT % CG == 0, "tile shape");
+    __shared__ uint4 lds[HALO > RED ? HALO : RED];
+    const int ng = NCT / CG;
+    const int bz = blockIdx.z / ng, ct0 = (blockIdx.z % ng) * CG;
+    stage_halo<HX, HY, HZ, CH>(x, X, Y, Z, blockIdx.x * kTX - 1, blockIdx.y * kTY - 1, bz - 1, lds);
+    stage_wait();
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    // B chunk of tile t at step s: this lane's base (voxel row r >> 3,
+    // column r & 7 of tile 0; channel chunk g) + tap offset + 2 t HX CH
+    const uint4* lb = lds + ((r >> 3) * HX + (r & 7)) * CH + g;
+    f4 acc[4][CG];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
+    const int s0 = wv * KW, s1 = s0 + KW < KSTEPS ? s0 + KW : KSTEPS;
+    const uint16_t* wr = w + ((ct0 * 16) + r) * KP + g * 8;
+    bf8 A[U][CG], An[U][CG];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < CG; ++c)
+            if (s0 + u < s1) An[u][c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP + (s0 + u) * 32);
+    for (int sg = s0; sg < s1; sg += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < CG; ++c) A[u][c] = An[u][c];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < CG; ++c)
+                if (sg + U + u < s1) An[u][c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP + (sg + U + u) * 32);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int st = sg + u;
+            if (st < s1) {
+                const int kb = st * 32, t0 = kb / CIN, cb = kb % CIN;
+                const uint4* pb = lb + (((t0 / 9) * HY + (t0 / 3) % 3) * HX + t0 % 3) * CH + cb / 8;
+                bf8 bf[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bf[t] = __builtin_bit_cast(bf8, pb[2 * t * HX * CH]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int c = 0; c < CG; ++c)
+                        acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[u][c], bf[t], acc[t][c], 0, 0, 0);
+            }
+        }
+    }
+    // partial sums: wave wv's tile t, group c at red[((t CG + c) 4 + wv) 64 + lane]
+    __syncthreads();  // every wave is done with the halo
+    f4* red = reinterpret_cast<f4*>(lds);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) red[((t * CG + c) * 4 + wv) * 64 + lane] = acc[t][c];
+    __syncthreads();
+    const int t = wv;  // this wave finishes tile wv
+    const int gx = blockIdx.x * kTX + (r & 7), gy = blockIdx.y * kTY + 2 * t + (r >> 3);
+    const int64_t vox = ((int64_t)bz * Y + gy) * X + gx;
+#pragma unroll
+    for (int c = 0; c < CG; ++c) {
+        const int o = (ct0 + c) * 16 + 4 * g;
+        // fixed order over the four K quarters
+        f4 sum = red[((t * CG + c) * 4 + 0) * 64 + lane];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) sum += red[((t * CG + c) * 4 + q) * 64 + lane];
+        if (o >= COUT) continue;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v[i] = sum[i] + bias[o + i];
+            if (relu) v[i] = fmaxf(v[i], 0.f);
+        }
+        *reinterpret_cast<uint2*>(y + vox * COUT + o) = make_uint2(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]));
     }
 }
 
@@ -200,29 +366,21 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
                                            : make_uint2(0u, 0u);
         }
     }
-    for (int q = threadIdx.x; q < HV * CH; q += kThreads) {
-        const int ch = q % CH, hv = q / CH;
-        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-        const int gx = cx0 - 1 + hx, gy = cy0 - 1 + hy, gz = cz0 - 1 + hz;
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (gx >= 0 && gx < S && gy >= 0 && gy < S && gz >= 0 && gz < S)
-            val = *reinterpret_cast<const uint4*>(d + (((int64_t)gz * S + gy) * S + gx) * CIN + ch * 8);
-        halo[q] = val;
-    }
+    stage_halo<HX, HY, HZ, CH>(d, S, S, S, cx0 - 1, cy0 - 1, cz0 - 1, halo);
+    stage_wait();
     __syncthreads();
     f4 acc[VPW][CG];
 #pragma unroll
     for (int a = 0; a < VPW; ++a)
 #pragma unroll
         for (int c = 0; c < CG; ++c) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
-    // the halo voxel of window offset 0 for this lane's coarse voxel
-    int hbase[VPW];
-#pragma unroll
-    for (int a = 0; a < VPW; ++a) {
-        const int p = 2 * wv + (a & 1);
-        const int px = p & 1, py = (p >> 1) & 1, pz = p >> 2;
-        hbase[a] = ((a >> 1) + pz) * HX * HY + (ly + py) * HX + (lx + px);
-    }
+    // The halo voxel of window offset 0 for this lane's coarse voxel in tile
+    // a (parity p = 2 wv + (a & 1): px = a & 1, py = wv & 1, pz = wv >> 1) is
+    // hb0 + (a >> 1) HX HY + (a & 1); with CIN a multiple of 32 the four k
+    // groups of a step are four channel chunks of one window offset, so every
+    // LDS read is this lane's base plus a compile-time offset.
+    const int hb0 = (wv >> 1) * HX * HY + (ly + (wv & 1)) * HX + lx;
+    const uint4* lb = halo + hb0 * CH + g;
     // A fragments of the wave's two parities, one step ahead
     const uint16_t* wr0 = w + ((int64_t)(2 * wv) * cpad(COUT) + ct0 * 16 + r) * KP + g * 8;
     const uint16_t* wr1 = wr0 + (int64_t)cpad(COUT) * KP;
@@ -233,9 +391,8 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
         an[1][c] = *reinterpret_cast<const bf8*>(wr1 + c * 16 * KP);
     }
     for (int s = 0; s < KSTEPS; ++s) {
-        const int k0 = s * 32 + g * 8;
-        const int e3 = k0 / CIN, c0 = k0 % CIN;
-        const int off = ((e3 >> 2) * HY + ((e3 >> 1) & 1)) * HX + (e3 & 1);
+        const int kb = s * 32, e3 = kb / CIN;  // window offset and chunk of k group 0
+        const uint4* pb = lb + (((e3 >> 2) * HY + ((e3 >> 1) & 1)) * HX + (e3 & 1)) * CH + (kb % CIN) / 8;
 #pragma unroll
         for (int c = 0; c < CG; ++c) {
             af[0][c] = an[0][c];
@@ -248,13 +405,14 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
                 an[1][c] = *reinterpret_cast<const bf8*>(wr1 + c * 16 * KP + (s + 1) * 32);
             }
         }
+        bf8 bf[VPW];
 #pragma unroll
-        for (int a = 0; a < VPW; ++a) {
-            const bf8 bf = __builtin_bit_cast(bf8, halo[(hbase[a] + off) * CH + c0 / 8]);
+        for (int a = 0; a < VPW; ++a) bf[a] = __builtin_bit_cast(bf8, pb[((a >> 1) * HX * HY + (a & 1)) * CH]);
+#pragma unroll
+        for (int a = 0; a < VPW; ++a)
 #pragma unroll
             for (int c = 0; c < CG; ++c)
-                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a & 1][c], bf, acc[a][c], 0, 0, 0);
-        }
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a & 1][c], bf[a], acc[a][c], 0, 0, 0);
     }
 #pragma unroll
     for (int a = 0; a < VPW; ++a) {
@@ -262,23 +420,22 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
         for (int c = 0; c < CG; ++c) {
             const int o = (ct0 + c) * 16 + 4 * g;
             if (o >= COUT) continue;
-            uint16_t h[4];
+            float t[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float t = acc[a][c][i] + bias[o + i];
-                if (relu && t < 0.f) t = 0.f;
-                h[i] = f2bf(t);
+                t[i] = acc[a][c][i] + bias[o + i];
+                if (relu) t[i] = fmaxf(t[i], 0.f);
             }
+            uint2 pk = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
             if (skip) {
+                // BF16(BF16(y) + skip): the rounding of a separate add
                 const uint2 sk = skv[a][c];
-                const uint16_t e[4] = {(uint16_t)(sk.x & 0xffffu), (uint16_t)(sk.x >> 16), (uint16_t)(sk.y & 0xffffu),
-                                       (uint16_t)(sk.y >> 16)};
-#pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = f2bf(bf2f(h[i]) + bf2f(e[i]));
+                const float u0 = __uint_as_float(pk.x << 16) + __uint_as_float(sk.x << 16);
+                const float u1 = __uint_as_float(pk.x & 0xffff0000u) + __uint_as_float(sk.x & 0xffff0000u);
+                const float u2 = __uint_as_float(pk.y << 16) + __uint_as_float(sk.y << 16);
+                const float u3 = __uint_as_float(pk.y & 0xffff0000u) + __uint_as_float(sk.y & 0xffff0000u);
+                pk = make_uint2(pack_bf2(u0, u1), pack_bf2(u2, u3));
             }
-            uint2 pk;
-            pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-            pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
             *reinterpret_cast<uint2*>(y + vox[a] * COUT + o) = pk;
         }
     }
@@ -774,14 +931,32 @@ __global__ void bf16_to_f32(const uint16_t* __restrict__ a, float* __restrict__ 
 inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // Block depth per input width: halo LDS stays within ~77 KB.
-constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin <= 32 ? 4 : (cin == 64 ? 2 : 1)); }
+#ifndef GELL_NN_TZ16
+#define GELL_NN_TZ16 8
+#endif
+#ifndef GELL_NN_TZ32
+#define GELL_NN_TZ32 4
+#endif
+constexpr int tz_for(int cin) { return cin <= 16 ? GELL_NN_TZ16 : (cin <= 32 ? GELL_NN_TZ32 : (cin == 64 ? 2 : 1)); }
 
+#ifndef GELL_NN_SPLITK_MAX
+#define GELL_NN_SPLITK_MAX 32
+#endif
 template <int CIN, int COUT>
 hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
                   hipStream_t st) {
     constexpr int TZ = tz_for(CIN);
     constexpr int NCT = cpad(COUT) / 16;
     if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
+    if constexpr (CIN % 32 == 0) {
+        // coarse levels (grids up to GELL_NN_SPLITK_MAX per side): K split over the waves
+        if (X <= GELL_NN_SPLITK_MAX && Y <= GELL_NN_SPLITK_MAX) {
+            constexpr int CGS = NCT % 2 == 0 ? 2 : 1;
+            hipLaunchKernelGGL((conv3_fwd_splitk<CIN, COUT, CGS>), dim3(X / kTX, Y / kTY, Z * (NCT / CGS)),
+                               dim3(kThreads), 0, st, x, w, b, y, X, Y, Z, relu);
+            return hipGetLastError();
+        }
+    }
     const int64_t base = (int64_t)(X / kTX) * (Y / kTY) * (Z / TZ);
     if (base >= 256 || NCT == 1)
         hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ, NCT>), dim3(X / kTX, Y / kTY, Z / TZ), dim3(kThreads), 0, st, x,
