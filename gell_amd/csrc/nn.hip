@@ -132,21 +132,26 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
     // wave wv is the 16 voxels (wv + 4 a) 16 .. + 15, i.e. z slice a, rows
     // 2 wv and 2 wv + 1; so fragment a sits PLANE chunks after fragment 0 and
     // every LDS read below is this lane's base plus a compile-time offset.
-    constexpr int PLANE = HX * HY * CH;
     const int hbase0 = (2 * wv + (r >> 3)) * HX + (r & 7);
     // B chunk of fragment a at step s: lane base (its voxel, and its k group's
     // channel chunk and tap shift) + tap offset of the step + a PLANE.
     // CIN >= 32: the four k groups of a step are four channel chunks of one
     // tap. CIN = 16: groups 0, 1 are the channel halves of tap 2 s, groups 2,
     // 3 those of tap 2 s + 1, whose offset from tap 2 s is one of three.
-    const uint4* lb1 = halo + hbase0 * CH + (CIN >= 32 ? g : (g & 1) + (g >> 1) * CH);
-    const uint4* lb2 = halo + hbase0 * CH + (g & 1) + (g >> 1) * CH * (HX - 2);
-    const uint4* lb3 = halo + hbase0 * CH + (g & 1) + (g >> 1) * CH * (HX * HY - 2 * HX - 2);
+    // chunk (ch, voxel v) at v CH + ch (a layout with each channel chunk's
+    // voxels 16 bytes apart measured no faster)
+    constexpr int PLANE = HX * HY * CH, VS = CH, CS = 1;
+    const uint4* lb1 = halo + hbase0 * VS + (CIN >= 32 ? g * CS : (g & 1) * CS + (g >> 1) * VS);
+    const uint4* lb2 = halo + hbase0 * VS + (g & 1) * CS + (g >> 1) * VS * (HX - 2);
+    const uint4* lb3 = halo + hbase0 * VS + (g & 1) * CS + (g >> 1) * VS * (HX * HY - 2 * HX - 2);
     const uint16_t* wr = w + ((ct0 * 16) + r) * KP + g * 8;
     // weight fragments of step s + 1 are loaded while step s computes
     bf8 af[CG], an[CG];
 #pragma unroll
     for (int c = 0; c < CG; ++c) an[c] = *reinterpret_cast<const bf8*>(wr + c * 16 * KP);
+    // two steps per iteration: enough B reads in flight ahead of the products
+    // without holding every step's fragments in registers
+#pragma unroll 2
     for (int s = 0; s < KSTEPS; ++s) {
         const int k0 = s * 32 + g * 8;
         const int tap = k0 / CIN, c0 = k0 % CIN;
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
         bf8 bf[VPW];
         if constexpr (CIN >= 32 || CIN == 16) {
             const int kb = s * 32, t0 = kb / CIN, cb = kb % CIN;  // tap and chunk of k group 0
-            const int toff = (((t0 / 9) * HY + (t0 / 3) % 3) * HX + t0 % 3) * CH + cb / 8;
+            const int toff = (((t0 / 9) * HY + (t0 / 3) % 3) * HX + t0 % 3) * VS + (cb / 8) * CS;
             const uint4* p = CIN >= 32 || t0 % 3 != 2 ? lb1 : ((t0 / 3) % 3 != 2 ? lb2 : lb3);
             const bool valid = CIN >= 32 || t0 + 1 < 27 || g < 2;  // the padding tap 27 reads zero
 #pragma unroll
@@ -174,7 +179,7 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
 #pragma unroll
             for (int a = 0; a < VPW; ++a)
                 bf[a] = __builtin_bit_cast(
-                    bf8, valid ? halo[(hbase0 + a * HX * HY + off) * CH + c0 / 8] : make_uint4(0, 0, 0, 0));
+                    bf8, valid ? halo[(hbase0 + a * HX * HY + off) * VS + (c0 / 8) * CS] : make_uint4(0, 0, 0, 0));
         }
 #pragma unroll
         for (int a = 0; a < VPW; ++a)
@@ -931,17 +936,8 @@ __global__ void bf16_to_f32(const uint16_t* __restrict__ a, float* __restrict__ 
 inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // Block depth per input width: halo LDS stays within ~77 KB.
-#ifndef GELL_NN_TZ16
-#define GELL_NN_TZ16 8
-#endif
-#ifndef GELL_NN_TZ32
-#define GELL_NN_TZ32 4
-#endif
-constexpr int tz_for(int cin) { return cin <= 16 ? GELL_NN_TZ16 : (cin <= 32 ? GELL_NN_TZ32 : (cin == 64 ? 2 : 1)); }
+constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin <= 32 ? 4 : (cin == 64 ? 2 : 1)); }
 
-#ifndef GELL_NN_SPLITK_MAX
-#define GELL_NN_SPLITK_MAX 32
-#endif
 template <int CIN, int COUT>
 hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
                   hipStream_t st) {
@@ -949,8 +945,8 @@ hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t
     constexpr int NCT = cpad(COUT) / 16;
     if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
     if constexpr (CIN % 32 == 0) {
-        // coarse levels (grids up to GELL_NN_SPLITK_MAX per side): K split over the waves
-        if (X <= GELL_NN_SPLITK_MAX && Y <= GELL_NN_SPLITK_MAX) {
+        // coarse levels (grids up to 32 per side): K split over the waves
+        if (X <= 32 && Y <= 32) {
             constexpr int CGS = NCT % 2 == 0 ? 2 : 1;
             hipLaunchKernelGGL((conv3_fwd_splitk<CIN, COUT, CGS>), dim3(X / kTX, Y / kTY, Z * (NCT / CGS)),
                                dim3(kThreads), 0, st, x, w, b, y, X, Y, Z, relu);

@@ -173,10 +173,12 @@ hipError_t up_layer(gell_surrogate* s, int l, const Tensor& coarse, Tensor& out,
 // per-parity weights are sums of the layer's FP32 taps rounded to BF16 once,
 // where the formed path rounds each tap, a difference of one BF16 rounding
 // per weight) and store the sum themselves.
-hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const float* rate, bool train) {
+// norms_ready: the norms of this (p0, p1) are already on the device.
+hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const float* rate, bool train,
+                   bool norms_ready = false) {
     if (!s->packed) SCHK(pack_all(s));
     hipStream_t st = s->stream;
-    SCHK(increment_norms(p0, p1, s->N, s->norm_part, s->norm, s->sums, st));
+    if (!norms_ready) SCHK(increment_norms(p0, p1, s->N, s->norm_part, s->norm, s->sums, st));
     SCHK(input_features(p0, p1, rate, s->N, s->norm, s->cfg.rate_scale, s->f0.d, st));
     SCHK(conv_layer(s, 0, s->f0, s->a0));
     SCHK(conv_layer(s, 1, s->a0, s->e1));
@@ -574,7 +576,7 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     if (s->ratio < 0.0 || s->since_check + 1 >= every) {
         // exact check: predict, then the other K - 1 real substeps, kept; the
         // prediction's error calibrates the estimate
-        e = forward(s, s->p0, field, ctx->rate[sub], false);
+        e = forward(s, s->p0, field, ctx->rate[sub], false, true);
         if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, s->pred, st);
         for (int k = 1; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
         double err = 0.0;
@@ -592,7 +594,7 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
         s->spent += est;
         *used = 1;
         ++s->n_accepted;
-        e = forward(s, s->p0, field, ctx->rate[sub], false);
+        e = forward(s, s->p0, field, ctx->rate[sub], false, true);
         // in place: the interior from p0 and the head, the outer layers stay p1
         if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, field, st);
     } else {
