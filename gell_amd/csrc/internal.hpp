@@ -154,6 +154,9 @@ inline int32_t grid_count(const gell_ctx* c, int32_t n) { return c->devn ? c->P.
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);
+// One substep from the field in src into dst (both N^3; src == dst: in place;
+// src may be the context's field and dst not, and the other way round).
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st);
 hipError_t xspan_rebuild(gell_ctx* ctx, int s, hipStream_t st);
 // Make the context stream wait for substrate s's own stream (no-op when
 // nothing is pending there); join_substrates does it for every substrate.

@@ -665,7 +665,7 @@ __global__ __launch_bounds__(64 * P) void lod_x_part(float* __restrict__ p, cons
 // [(k P + block) N + i] for the y line (i, k).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const float* __restrict__ rate,
+__global__ __launch_bounds__(N) void lod_xy_part(const float* src, float* dst, const float* __restrict__ rate,
                                                  const int2* __restrict__ xspan, const float* __restrict__ tab,
                                                  float side, float bnd, float dt, float* __restrict__ coup) {
     constexpr int P = N / 64;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const fl
     const int64_t L0 = (int64_t)blockIdx.x * 64;  // x lines (j, k), j = L % N
     const int kz = (int)(L0 / N), yb = (int)((L0 % N) / 64);
     float v[m];
-    x_lines_solve<N, P>(v, p, rate, xspan, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
+    x_lines_solve<N, P>(v, src, rate, xspan, tab, side, bnd, dt, L0, kx, lane, smem + kx * 64 * kTileLd, bt, bb);
     // Column i = kx m + lane of the block comes from this wave's own 64 x 64
     // segment (row r = lane r), so each wave transposes it through its own
     // 64 x 33 tile in two halves of 32 columns. (Every wave is past its
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(N) void lod_xy_part(float* __restrict__ p, const fl
     part_local<m>(v, tab + yb * m, tab + N + yb * m, side);
     coup[(kz * P + yb) * N + i] = v[0];
     coup[P * NN + (kz * P + yb) * N + i] = v[m - 1];
-    const __amdgpu_buffer_rsrc_t rs = field_rsrc(p, NN * N);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(dst, NN * N);
     const int so0 = (int)(L0 * N * 4);
 #pragma unroll
     for (int t = 0; t < m; ++t)
@@ -972,13 +972,15 @@ void launch_z_ycorr(float* p, const float* tabz, const float* taby, const float*
 // Fused substep (see lod_xy_part); zp = z blocks, falling back to the next
 // smaller count the grid supports.
 template <int N>
-hipError_t launch_fused(float* p, const float* rate, const int2* xspan, float* const* tabs, float side, float bnd,
-                        float dt, int zp, float* coup, hipStream_t st) {
+hipError_t launch_fused(const float* src, float* p, const float* rate, const int2* xspan, float* const* tabs,
+                        float side, float bnd, float dt, int zp, float* coup, hipStream_t st) {
     constexpr int P = N / 64;
     static_assert(P == 2 || P == 4 || P == 8, "fused substep needs N in {128, 256, 512}");
     const float* taby = tabs[P == 2 ? 0 : (P == 4 ? 1 : 2)];
-    hipLaunchKernelGGL((lod_xy_part<N>), dim3((unsigned)((int64_t)N * N / 64)), dim3(N), 0, st, p, rate, xspan, taby,
-                       side, bnd, dt, coup);
+    // the x + y kernel reads src and writes p (src == p: in place); the z
+    // sweep then works on p
+    hipLaunchKernelGGL((lod_xy_part<N>), dim3((unsigned)((int64_t)N * N / 64)), dim3(N), 0, st, src, p, rate, xspan,
+                       taby, side, bnd, dt, coup);
     hipLaunchKernelGGL((lod_y_reduce<N>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0, st, taby,
                        coup);
     constexpr bool ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0, ok16 = N % (16 * kChunk) == 0;
@@ -1002,8 +1004,15 @@ bool lod_fused_active(const gell_ctx* ctx) {
 }
 
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
+    return lod_substep_launch(ctx, s, ctx->field[s], ctx->field[s], st);
+}
+
+// One substep from the field in src to the field in dst (src == dst: in
+// place). The fused substep reads src in its first kernel; the other forms
+// copy src to dst first and work in place on dst.
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st) {
     const int N = ctx->N;
-    float* p = ctx->field[s];
+    float* p = dst;
     const float* rate = ctx->rate[s];
     const float* E = ctx->E[s];
     const float* F = ctx->F[s];
@@ -1021,9 +1030,13 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
     }
     if (lod_fused_active(ctx)) {
         float* const* tabs = ctx->ptab[s];
-        if (N == 128) return launch_fused<128>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        if (N == 256) return launch_fused<256>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        return launch_fused<512>(p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 128) return launch_fused<128>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 256) return launch_fused<256>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        return launch_fused<512>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+    }
+    if (src != p) {
+        const hipError_t e = hipMemcpyAsync(p, src, sizeof(float) * (size_t)ctx->V, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
     }
     if ((xp > 1 || yzp > 1) && (N == 128 || N == 192 || N == 256 || N == 512)) {
         float* const* tabs = ctx->ptab[s];

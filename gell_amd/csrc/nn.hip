@@ -843,9 +843,10 @@ __global__ void features(const float* __restrict__ p0, const float* __restrict__
 // Full-resolution output: out[fine] = p0[fine] + s out_scale head[coarse][sub],
 // sub = (zi & 1) 4 + (yi & 1) 2 + (xi & 1), in the interior; the two outer
 // layers, which every substep first resets to the boundary value, take p1
-// (out may alias p1). One thread per coarse voxel: its 8 head values in one
-// 16-byte load, its fine voxels as 4 x-pairs.
-__global__ void head_out(const float* __restrict__ p0, const float* p1, const uint16_t* __restrict__ head, int N,
+// (out may alias p0 or p1: each thread reads a voxel before writing it).
+// One thread per coarse voxel: its 8 head values in one 16-byte load, its
+// fine voxels as 4 x-pairs.
+__global__ void head_out(const float* p0, const float* p1, const uint16_t* __restrict__ head, int N,
                          const float* __restrict__ norm, float out_scale, float* out) {
     const int X = N / 2;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -88,12 +88,13 @@ struct gell_surrogate {
     float* loss_part = nullptr;
     float* h_red = nullptr;  // pinned
     float* pred = nullptr;   // N^3 scratch for advance: the prediction at a check
-    float* p0 = nullptr;     // N^3 scratch for advance: the field before the call
+    float* p_first = nullptr;  // N^3 scratch for advance: the field after its first substep
     float* red_part = nullptr;
     double* norm_part = nullptr;  // increment_norms partials
     float* norm = nullptr;        // s, 1 / s (device)
     double* sums = nullptr;       // sum r^2 (interior), sum p0^2 (device)
     double* h_sums = nullptr;     // pinned copy of sums
+    hipEvent_t sums_ready = nullptr;  // recorded after the copy of sums to h_sums
     int adam_t = 0;
     int accumulate = 0;  // add this call's gradients to the stored ones
     // The in-loop gate (gell_surrogate_advance): the largest measured ratio
@@ -103,6 +104,7 @@ struct gell_surrogate {
     // of each kind of call.
     double ratio = -1.0;
     int since_check = 0;
+    bool last_kept = false;  // the last call between checks kept its prediction
     double spent = 0.0;
     int64_t n_checked = 0, n_accepted = 0, n_fallback = 0;
     std::vector<void*> allocs;
@@ -412,13 +414,14 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     SA(salloc(s, &s->wg_out, wgo));
     SA(salloc(s, &s->loss_part, (int64_t)S1 * S1 * S1 / 256 + 1));
     SA(salloc(s, &s->pred, (int64_t)s->N * s->N * s->N));
-    SA(salloc(s, &s->p0, (int64_t)s->N * s->N * s->N));
+    SA(salloc(s, &s->p_first, (int64_t)s->N * s->N * s->N));
     SA(salloc(s, &s->red_part, 2 * kRedBlocks));
     SA(salloc(s, &s->norm_part, 2 * kNormBlocks));
     SA(salloc(s, &s->norm, 2));
     SA(salloc(s, &s->sums, 2));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_red), sizeof(float) * (2 * kRedBlocks + 8));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_sums), sizeof(double) * 2);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sums_ready, hipEventDisableTiming);
 #undef SA
     if (e == hipSuccess) {
         // He-normal initialisation from a seeded host generator; the head
@@ -452,6 +455,7 @@ void gell_surrogate_destroy(gell_surrogate* s) {
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->h_red) (void)hipHostFree(s->h_red);
     if (s->h_sums) (void)hipHostFree(s->h_sums);
+    if (s->sums_ready) (void)hipEventDestroy(s->sums_ready);
     delete s;
 }
 
@@ -552,7 +556,10 @@ hipError_t rel_change(gell_surrogate* s, const float* a, const float* b, int64_t
 // starts a new one), stays within tol; otherwise the other K - 1 substeps
 // run and the network does not. The error a window adds to the field is thus
 // bounded by tol as far as the calibrated estimate holds; tol = 0 makes every
-// call exact.
+// call exact. The first substep runs out of place (p0 stays in the field, p1
+// in a scratch field, from which the second substep of an exact run writes
+// back), so no copy of the field is made; on checks, and after a call that
+// kept its prediction, the network is queued before the host reads rho.
 int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float tol, int32_t* used, float* residual) {
     if (!s || !ctx || !used) return sfail(s, GELL_E_ARG, "null argument");
     if (ctx->N != s->N) return sfail(s, GELL_E_ARG, "grid size differs from the surrogate's");
@@ -560,25 +567,40 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     if (ctx->stream != s->stream) return sfail(s, GELL_E_ARG, "surrogate and context must share a stream");
     hipStream_t st = s->stream;
     const int64_t V = ctx->V;
-    float* const field = ctx->field[sub];
+    float* const field = ctx->field[sub];  // p0 until the call's end
+    const float* const rate = ctx->rate[sub];
     *used = 0;
-    // keep p0, run the first substep for real (field = p1), measure its increment
+    const int every = s->cfg.check_every > 0 ? s->cfg.check_every : 1;
+    const bool check = s->ratio < 0.0 || s->since_check + 1 >= every;
+    // The network runs before the host has read rho when the call is a check
+    // (it always predicts) or when the last call kept its prediction (it
+    // likely will again); a call that then falls back has spent it for
+    // nothing.
+    const bool early = check || (tol > 0.f && s->last_kept);
+    // the first substep for real, out of place: p0 stays in the field, p1
+    // goes to the scratch; then its increment's norms, read back while the
+    // network runs
     hipError_t e = gell::join_substrates(ctx);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->p0, field, sizeof(float) * V, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = gell::lod_substep_launch(ctx, sub, st);
-    if (e == hipSuccess) e = increment_norms(s->p0, field, s->N, s->norm_part, s->norm, s->sums, st);
+    if (e == hipSuccess) e = gell::lod_substep_launch(ctx, sub, field, s->p_first, st);
+    if (e == hipSuccess) e = increment_norms(field, s->p_first, s->N, s->norm_part, s->norm, s->sums, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->h_sums, s->sums, sizeof(double) * 2, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipEventRecord(s->sums_ready, st);
+    if (e == hipSuccess && early) e = forward(s, field, s->p_first, rate, false, true);
+    if (e == hipSuccess) e = hipEventSynchronize(s->sums_ready);
     if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
     const double rho = s->h_sums[1] > 0.0 ? std::sqrt(s->h_sums[0] / s->h_sums[1]) : 0.0;
     if (residual) *residual = (float)rho;
-    const int every = s->cfg.check_every > 0 ? s->cfg.check_every : 1;
-    if (s->ratio < 0.0 || s->since_check + 1 >= every) {
-        // exact check: predict, then the other K - 1 real substeps, kept; the
-        // prediction's error calibrates the estimate
-        e = forward(s, s->p0, field, ctx->rate[sub], false, true);
-        if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, s->pred, st);
-        for (int k = 1; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
+    // the other K - 1 substeps: the second from p1 back into the field
+    const auto exact_rest = [&]() {
+        hipError_t r = gell::lod_substep_launch(ctx, sub, s->p_first, field, st);
+        for (int k = 2; k < s->cfg.K && r == hipSuccess; ++k) r = gell::lod_substep_launch(ctx, sub, st);
+        return r;
+    };
+    if (check) {
+        // exact check: the prediction, then the other K - 1 real substeps,
+        // kept; the prediction's error calibrates the estimate
+        e = output_field(field, s->p_first, s->h.d, s->N, s->norm, s->cfg.out_scale, s->pred, st);
+        if (e == hipSuccess) e = exact_rest();
         double err = 0.0;
         if (e == hipSuccess) e = rel_change(s, field, s->pred, V, &err);
         if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance check: ") + hipGetErrorString(e));
@@ -594,13 +616,14 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
         s->spent += est;
         *used = 1;
         ++s->n_accepted;
-        e = forward(s, s->p0, field, ctx->rate[sub], false, true);
-        // in place: the interior from p0 and the head, the outer layers stay p1
-        if (e == hipSuccess) e = output_field(s->p0, field, s->h.d, s->N, s->norm, s->cfg.out_scale, field, st);
+        if (!early) e = forward(s, field, s->p_first, rate, false, true);
+        // in place: the interior from p0 and the head, the outer layers from p1
+        if (e == hipSuccess) e = output_field(field, s->p_first, s->h.d, s->N, s->norm, s->cfg.out_scale, field, st);
     } else {
         ++s->n_fallback;
-        for (int k = 1; k < s->cfg.K && e == hipSuccess; ++k) e = gell::lod_substep_launch(ctx, sub, st);
+        e = exact_rest();
     }
+    s->last_kept = *used != 0;
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
 }
 

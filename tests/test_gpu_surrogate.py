@@ -63,9 +63,16 @@ def _twin(P, ctx, buf):
     return t, tb
 
 
-def test_tol0_gives_the_lod_substeps_bit_for_bit():
+@pytest.mark.parametrize("fused", [1, 0])
+def test_tol0_gives_the_lod_substeps_bit_for_bit(fused):
+    """Also pins the call's out-of-place substeps (the first from the field
+    into a scratch field, the second back): with the fused substep its first
+    kernel reads one field and writes the other; otherwise the field is
+    copied and the substep runs in place."""
     P, ctx, buf = _state(100_000, 11)
     twin, tbuf = _twin(P, ctx, buf)
+    for c in (ctx, twin):
+        c.tune("lod_fused", fused)
     sur = rt.Surrogate(N, K)
     sur.set_params(weights())
     for call in range(3):
