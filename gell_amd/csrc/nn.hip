@@ -90,6 +90,17 @@ __device__ __forceinline__ void stage_halo(const uint16_t* base, int X, int Y, i
 __device__ __forceinline__ void stage_wait() { __builtin_amdgcn_s_waitcnt(0); }
 constexpr int kTX = 8, kTY = 8;
 
+// The network's output written by its last layer (the head) instead of a
+// BF16 head tensor: out at the fine voxels of each coarse voxel, as head_out
+// forms it (see there); `out` may alias p0 or p1.
+struct HeadOut {
+    const float* p0;
+    const float* p1;
+    const float* norm;
+    float out_scale;
+    float* out;
+};
+
 // ---------------------------------------------------------------------------
 // Forward convolution. Block = 8 x 8 x TZ output voxels and CG of the layer's
 // 16-channel output tiles (blockIdx.z also walks the NCT / CG tile groups);
@@ -102,10 +113,10 @@ constexpr int kTX = 8, kTY = 8;
 // loads per wave.
 // Epilogue: + bias, ReLU (when relu), BF16 store.
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int TZ, int CG>
+template <int CIN, int COUT, int TZ, int CG, bool HEAD = false>
 __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                       const float* __restrict__ bias, uint16_t* __restrict__ y, int X,
-                                                      int Y, int Z, int relu) {
+                                                      int Y, int Z, int relu, HeadOut ho = HeadOut{}) {
     constexpr int HX = kTX + 2, HY = kTY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int CH = CIN / 8;
     constexpr int KP = kpad(CIN);
@@ -203,7 +214,32 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
                 t[i] = acc[a][c][i] + bias[o + i];
                 if (relu) t[i] = fmaxf(t[i], 0.f);
             }
-            *reinterpret_cast<uint2*>(y + vox * COUT + o) = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
+            const uint2 pk = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
+            if constexpr (HEAD) {
+                // head channels o .. o + 3 are fine voxels sub = o + i of
+                // coarse voxel (gx, gy, gz): z parity o >> 2, (y, x) parity
+                // (i >> 1, i & 1); the BF16-rounded head value, as head_out
+                // reads it from the head tensor
+                const int N = 2 * X;
+                const float scale = ho.norm[0] * ho.out_scale;
+                const float h[4] = {__uint_as_float(pk.x << 16), __uint_as_float(pk.x & 0xffff0000u),
+                                    __uint_as_float(pk.y << 16), __uint_as_float(pk.y & 0xffff0000u)};
+                const int zi = 2 * gz + (o >> 2);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int yi = 2 * gy + q, x0 = 2 * gx, x1 = 2 * gx + 1;
+                    const int64_t fo = ((int64_t)zi * N + yi) * N + x0;
+                    const float2 pa = *reinterpret_cast<const float2*>(ho.p0 + fo);
+                    const float2 pb = *reinterpret_cast<const float2*>(ho.p1 + fo);
+                    const bool row_edge = yi < 2 || zi < 2 || yi >= N - 2 || zi >= N - 2;
+                    float2 rv;
+                    rv.x = (row_edge || x0 < 2 || x0 >= N - 2) ? pb.x : pa.x + scale * h[2 * q];
+                    rv.y = (row_edge || x1 < 2 || x1 >= N - 2) ? pb.y : pa.y + scale * h[2 * q + 1];
+                    *reinterpret_cast<float2*>(ho.out + fo) = rv;
+                }
+            } else {
+                *reinterpret_cast<uint2*>(y + vox * COUT + o) = pk;
+            }
         }
     }
 }
@@ -1115,6 +1151,17 @@ hipError_t output_field(const float* p0, const float* p1, const uint16_t* head, 
                         float out_scale, float* out, hipStream_t st) {
     hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)(N / 2) * (N / 2) * (N / 2))), dim3(256), 0, st, p0, p1,
                        head, N, norm, out_scale, out);
+    return hipGetLastError();
+}
+
+hipError_t conv3_head_out(const uint16_t* x, const uint16_t* w, const float* b, const float* p0, const float* p1,
+                          int N, const float* norm, float out_scale, float* out, hipStream_t st) {
+    constexpr int TZ = tz_for(16);
+    const int X = N / 2;
+    if (X % kTX || X % TZ) return hipErrorInvalidValue;
+    const HeadOut ho{p0, p1, norm, out_scale, out};
+    hipLaunchKernelGGL((conv3_fwd<16, 8, TZ, 1, true>), dim3(X / kTX, X / kTY, X / TZ), dim3(kThreads), 0, st, x, w, b,
+                       nullptr, X, X, X, 0, ho);
     return hipGetLastError();
 }
 

@@ -62,6 +62,12 @@ hipError_t input_features(const float* p0, const float* p1, const float* rate, i
 // the two outer layers (sub-voxel order z, y, x); out may alias p1
 hipError_t output_field(const float* p0, const float* p1, const uint16_t* head, int N, const float* norm,
                         float out_scale, float* out, hipStream_t st);
+// The head layer (16 -> 8 channels on the (N/2)^3 grid of x, no ReLU) with
+// output_field fused into it: out (N^3) from p0, p1 and the BF16-rounded
+// head values, bit for bit what conv3 into a head tensor + output_field give;
+// out may alias p0 or p1.
+hipError_t conv3_head_out(const uint16_t* x, const uint16_t* w, const float* b, const float* p0, const float* p1,
+                          int N, const float* norm, float out_scale, float* out, hipStream_t st);
 // mean squared error of the increment in units of s out_scale over the
 // interior; dL/dhead and per-block loss partial sums (*nparts of them)
 hipError_t loss_head(const float* p0, const uint16_t* head, const float* target, int N, const float* norm,

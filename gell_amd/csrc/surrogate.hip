@@ -18,16 +18,18 @@
 //
 //   features(r / s, rate) 24 ch: r / s (as BF16 high + low parts) and rate
 //                   at the 8 fine voxels of each coarse one
-//   level 1 (N/2):  L0 24->16, L1 16->16 = e1             pool
-//   level 2 (N/4):  L2 16->32, L3 32->32 = e2             pool
-//   level 3 (N/8):  L4 32->64, L5 64->64 = e3             pool
-//   level 4 (N/16): L6 64->128, L7 128->128 = e4          up
-//   level 3:        L8 128->64 (+ e3), L9 64->64 = d3     up
-//   level 2:        L10 64->32 (+ e2), L11 32->32 = d2    up
-//   level 1:        L12 32->16 (+ e1), L13 16->16 = d1
-//   head:           L14 16->8, no ReLU: the 2x2x2 fine voxels of each coarse
+//   level 1 (N/2):  L0 24->16 = e1                        pool
+//   level 2 (N/4):  L1 16->32, L2 32->32 = e2             pool
+//   level 3 (N/8):  L3 32->64, L4 64->64 = e3             pool
+//   level 4 (N/16): L5 64->128, L6 128->128 = e4          up
+//   level 3:        L7 128->64 (+ e3), L8 64->64 = d3     up
+//   level 2:        L9 64->32 (+ e2), L10 32->32 = d2     up
+//   level 1:        L11 32->16 (+ e1) = x1
+//   head:           L12 16->8, no ReLU: the 2x2x2 fine voxels of each coarse
 //                   voxel, added to p0 (in units of s out_scale); the two
 //                   outer layers, which every substep resets first, take p1.
+// Level 1 holds one convolution on each side (the other levels two): its
+// 16-channel layers on the (N/2)^3 grid cost the most per layer.
 //
 // Every layer but the head applies ReLU; the "+ e" skips are added after it.
 // Training (gradients + Adam) runs on the same kernels: input gradients are
@@ -51,11 +53,11 @@ struct LayerSpec {
 };
 
 const LayerSpec kLayers[] = {
-    {24, 16, 1, true},   {16, 16, 1, true},  {16, 32, 2, true},  {32, 32, 2, true},  {32, 64, 3, true},
-    {64, 64, 3, true},   {64, 128, 4, true}, {128, 128, 4, true}, {128, 64, 3, true}, {64, 64, 3, true},
-    {64, 32, 2, true},   {32, 32, 2, true},  {32, 16, 1, true},  {16, 16, 1, true},  {16, 8, 1, false},
+    {24, 16, 1, true},  {16, 32, 2, true},   {32, 32, 2, true},  {32, 64, 3, true}, {64, 64, 3, true},
+    {64, 128, 4, true}, {128, 128, 4, true}, {128, 64, 3, true}, {64, 64, 3, true}, {64, 32, 2, true},
+    {32, 32, 2, true},  {32, 16, 1, true},   {16, 8, 1, false},
 };
-constexpr int kNL = 15;
+constexpr int kNL = 13;
 
 struct Tensor {
     uint16_t* d = nullptr;
@@ -79,7 +81,7 @@ struct gell_surrogate {
     bool packed = false;     // wb / wf current
     float* zero_bias = nullptr;
     // forward tensors
-    Tensor f0, a0, e1, p1, a2, e2, p2, a4, e3, p3, a6, e4, u3, c3, x3, d3, u2, c2, x2, d2, u1, c1, x1, d1, h;
+    Tensor f0, e1, p1, a2, e2, p2, a4, e3, p3, a6, e4, u3, c3, x3, d3, u2, c2, x2, d2, u1, c1, x1, h;
     // gradient scratch: two per level + head
     Tensor g1a, g1b, g1c, g2a, g2b, g2c, g3a, g3b, g3c, g4a, g4b, dh;
     float* wg_scratch = nullptr;
@@ -140,8 +142,8 @@ hipError_t tensor(gell_surrogate* s, Tensor& t, int S, int C) {
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
-// The layers whose input is an upsampled tensor (8, 10, 12).
-bool after_up(int l) { return l == 8 || l == 10 || l == 12; }
+// The layers whose input is an upsampled tensor (7, 9, 11).
+bool after_up(int l) { return l == 7 || l == 9 || l == 11; }
 
 hipError_t pack_all(gell_surrogate* s) {
     for (int l = 0; l < kNL; ++l) {
@@ -175,49 +177,53 @@ hipError_t up_layer(gell_surrogate* s, int l, const Tensor& coarse, Tensor& out,
 // per-parity weights are sums of the layer's FP32 taps rounded to BF16 once,
 // where the formed path rounds each tap, a difference of one BF16 rounding
 // per weight) and store the sum themselves.
-// norms_ready: the norms of this (p0, p1) are already on the device.
+// norms_ready: the norms of this (p0, p1) are already on the device. With
+// out (inference only) the head layer writes the predicted field there
+// itself (conv3_head_out); otherwise it writes the head tensor h.
 hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const float* rate, bool train,
-                   bool norms_ready = false) {
+                   bool norms_ready = false, float* out = nullptr) {
     if (!s->packed) SCHK(pack_all(s));
     hipStream_t st = s->stream;
     if (!norms_ready) SCHK(increment_norms(p0, p1, s->N, s->norm_part, s->norm, s->sums, st));
     SCHK(input_features(p0, p1, rate, s->N, s->norm, s->cfg.rate_scale, s->f0.d, st));
-    SCHK(conv_layer(s, 0, s->f0, s->a0));
-    SCHK(conv_layer(s, 1, s->a0, s->e1));
+    SCHK(conv_layer(s, 0, s->f0, s->e1));
     SCHK(pool(s->e1.d, s->p1.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
-    SCHK(conv_layer(s, 2, s->p1, s->a2));
-    SCHK(conv_layer(s, 3, s->a2, s->e2));
+    SCHK(conv_layer(s, 1, s->p1, s->a2));
+    SCHK(conv_layer(s, 2, s->a2, s->e2));
     SCHK(pool(s->e2.d, s->p2.d, s->p2.S, s->p2.S, s->p2.S, s->p2.C, st));
-    SCHK(conv_layer(s, 4, s->p2, s->a4));
-    SCHK(conv_layer(s, 5, s->a4, s->e3));
+    SCHK(conv_layer(s, 3, s->p2, s->a4));
+    SCHK(conv_layer(s, 4, s->a4, s->e3));
     SCHK(pool(s->e3.d, s->p3.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));
-    SCHK(conv_layer(s, 6, s->p3, s->a6));
-    SCHK(conv_layer(s, 7, s->a6, s->e4));
+    SCHK(conv_layer(s, 5, s->p3, s->a6));
+    SCHK(conv_layer(s, 6, s->a6, s->e4));
     if (train) {
         SCHK(upsample(s->e4.d, s->u3.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));
-        SCHK(conv_layer(s, 8, s->u3, s->c3));
+        SCHK(conv_layer(s, 7, s->u3, s->c3));
         SCHK(add(s->c3.d, s->e3.d, s->x3.d, s->x3.n(), st));
     } else {
-        SCHK(up_layer(s, 8, s->e4, s->x3, s->e3.d));
+        SCHK(up_layer(s, 7, s->e4, s->x3, s->e3.d));
     }
-    SCHK(conv_layer(s, 9, s->x3, s->d3));
+    SCHK(conv_layer(s, 8, s->x3, s->d3));
     if (train) {
         SCHK(upsample(s->d3.d, s->u2.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));
-        SCHK(conv_layer(s, 10, s->u2, s->c2));
+        SCHK(conv_layer(s, 9, s->u2, s->c2));
         SCHK(add(s->c2.d, s->e2.d, s->x2.d, s->x2.n(), st));
     } else {
-        SCHK(up_layer(s, 10, s->d3, s->x2, s->e2.d));
+        SCHK(up_layer(s, 9, s->d3, s->x2, s->e2.d));
     }
-    SCHK(conv_layer(s, 11, s->x2, s->d2));
+    SCHK(conv_layer(s, 10, s->x2, s->d2));
     if (train) {
         SCHK(upsample(s->d2.d, s->u1.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));
-        SCHK(conv_layer(s, 12, s->u1, s->c1));
+        SCHK(conv_layer(s, 11, s->u1, s->c1));
         SCHK(add(s->c1.d, s->e1.d, s->x1.d, s->x1.n(), st));
     } else {
-        SCHK(up_layer(s, 12, s->d2, s->x1, s->e1.d));
+        SCHK(up_layer(s, 11, s->d2, s->x1, s->e1.d));
     }
-    SCHK(conv_layer(s, 13, s->x1, s->d1));
-    SCHK(conv_layer(s, 14, s->d1, s->h));
+    if (out && !train)
+        SCHK(conv3_head_out(s->x1.d, s->wb[12], s->param + s->b_off[12], p0, p1, s->N, s->norm, s->cfg.out_scale,
+                            out, st));
+    else
+        SCHK(conv_layer(s, 12, s->x1, s->h));
     return hipSuccess;
 }
 
@@ -238,51 +244,47 @@ hipError_t layer_back(gell_surrogate* s, int l, const Tensor& in, const Tensor& 
 hipError_t backward(gell_surrogate* s) {
     hipStream_t st = s->stream;
     // head (no ReLU): dh holds dL/dh
-    SCHK(layer_back(s, 14, s->d1, s->dh, &s->g1a));                 // g1a = dd1
-    SCHK(relu_back(s->g1a.d, s->d1.d, s->g1a.n(), st));
-    SCHK(layer_back(s, 13, s->x1, s->g1a, &s->g1b));                // g1b = dx1 (= dc1, part of de1)
+    SCHK(layer_back(s, 12, s->x1, s->dh, &s->g1b));                 // g1b = dx1 (= dc1, part of de1)
     SCHK(hipMemcpyAsync(s->g1c.d, s->g1b.d, s->g1b.n() * 2, hipMemcpyDeviceToDevice, st));  // g1c = de1 so far
     SCHK(relu_back(s->g1b.d, s->c1.d, s->g1b.n(), st));
-    // The input gradients of L12, L10, L8, L6, L4 and L2 have the shape of
+    // The input gradients of L11, L9, L7, L5, L3 and L1 have the shape of
     // the layer's own input (u1, u2, u3, p3, p2, p1), which the backward pass
     // no longer needs once that layer's weight gradient is taken: they are
     // written in place of it.
-    SCHK(layer_back(s, 12, s->u1, s->g1b, &s->u1));                 // u1 <- du1
+    SCHK(layer_back(s, 11, s->u1, s->g1b, &s->u1));                 // u1 <- du1
     SCHK(upsample_back(s->u1.d, s->g2a.d, s->d2.S, s->d2.S, s->d2.S, s->d2.C, st));  // g2a = dd2
     SCHK(relu_back(s->g2a.d, s->d2.d, s->g2a.n(), st));
-    SCHK(layer_back(s, 11, s->x2, s->g2a, &s->g2b));                // g2b = dx2
+    SCHK(layer_back(s, 10, s->x2, s->g2a, &s->g2b));                // g2b = dx2
     SCHK(hipMemcpyAsync(s->g2c.d, s->g2b.d, s->g2b.n() * 2, hipMemcpyDeviceToDevice, st));  // g2c = de2 so far
     SCHK(relu_back(s->g2b.d, s->c2.d, s->g2b.n(), st));
-    SCHK(layer_back(s, 10, s->u2, s->g2b, &s->u2));                 // u2 <- du2 (64 ch, level 2)
+    SCHK(layer_back(s, 9, s->u2, s->g2b, &s->u2));                  // u2 <- du2 (64 ch, level 2)
     SCHK(upsample_back(s->u2.d, s->g3a.d, s->d3.S, s->d3.S, s->d3.S, s->d3.C, st));  // g3a = dd3
     SCHK(relu_back(s->g3a.d, s->d3.d, s->g3a.n(), st));
-    SCHK(layer_back(s, 9, s->x3, s->g3a, &s->g3b));                 // g3b = dx3
+    SCHK(layer_back(s, 8, s->x3, s->g3a, &s->g3b));                 // g3b = dx3
     SCHK(hipMemcpyAsync(s->g3c.d, s->g3b.d, s->g3b.n() * 2, hipMemcpyDeviceToDevice, st));  // g3c = de3 so far
     SCHK(relu_back(s->g3b.d, s->c3.d, s->g3b.n(), st));
-    SCHK(layer_back(s, 8, s->u3, s->g3b, &s->u3));                  // u3 <- du3 (128 ch, level 3)
+    SCHK(layer_back(s, 7, s->u3, s->g3b, &s->u3));                  // u3 <- du3 (128 ch, level 3)
     SCHK(upsample_back(s->u3.d, s->g4a.d, s->e4.S, s->e4.S, s->e4.S, s->e4.C, st));  // g4a = de4
     SCHK(relu_back(s->g4a.d, s->e4.d, s->g4a.n(), st));
-    SCHK(layer_back(s, 7, s->a6, s->g4a, &s->g4b));                 // g4b = da6
+    SCHK(layer_back(s, 6, s->a6, s->g4a, &s->g4b));                 // g4b = da6
     SCHK(relu_back(s->g4b.d, s->a6.d, s->g4b.n(), st));
-    SCHK(layer_back(s, 6, s->p3, s->g4b, &s->p3));                  // p3 <- dp3 (64 ch, level 4)
+    SCHK(layer_back(s, 5, s->p3, s->g4b, &s->p3));                  // p3 <- dp3 (64 ch, level 4)
     SCHK(pool_back(s->p3.d, s->g3a.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));  // g3a = pool part of de3
     SCHK(add(s->g3a.d, s->g3c.d, s->g3c.d, s->g3c.n(), st));        // g3c = de3
     SCHK(relu_back(s->g3c.d, s->e3.d, s->g3c.n(), st));
-    SCHK(layer_back(s, 5, s->a4, s->g3c, &s->g3a));                 // g3a = da4
+    SCHK(layer_back(s, 4, s->a4, s->g3c, &s->g3a));                 // g3a = da4
     SCHK(relu_back(s->g3a.d, s->a4.d, s->g3a.n(), st));
-    SCHK(layer_back(s, 4, s->p2, s->g3a, &s->p2));                  // p2 <- dp2 (32 ch, level 3)
+    SCHK(layer_back(s, 3, s->p2, s->g3a, &s->p2));                  // p2 <- dp2 (32 ch, level 3)
     SCHK(pool_back(s->p2.d, s->g2a.d, s->p2.S, s->p2.S, s->p2.S, s->p2.C, st));
     SCHK(add(s->g2a.d, s->g2c.d, s->g2c.d, s->g2c.n(), st));        // g2c = de2
     SCHK(relu_back(s->g2c.d, s->e2.d, s->g2c.n(), st));
-    SCHK(layer_back(s, 3, s->a2, s->g2c, &s->g2a));                 // g2a = da2
+    SCHK(layer_back(s, 2, s->a2, s->g2c, &s->g2a));                 // g2a = da2
     SCHK(relu_back(s->g2a.d, s->a2.d, s->g2a.n(), st));
-    SCHK(layer_back(s, 2, s->p1, s->g2a, &s->p1));                  // p1 <- dp1 (16 ch, level 2)
+    SCHK(layer_back(s, 1, s->p1, s->g2a, &s->p1));                  // p1 <- dp1 (16 ch, level 2)
     SCHK(pool_back(s->p1.d, s->g1a.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
     SCHK(add(s->g1a.d, s->g1c.d, s->g1c.d, s->g1c.n(), st));        // g1c = de1
     SCHK(relu_back(s->g1c.d, s->e1.d, s->g1c.n(), st));
-    SCHK(layer_back(s, 1, s->a0, s->g1c, &s->g1a));                 // g1a = da0
-    SCHK(relu_back(s->g1a.d, s->a0.d, s->g1a.n(), st));
-    SCHK(layer_back(s, 0, s->f0, s->g1a, nullptr));
+    SCHK(layer_back(s, 0, s->f0, s->g1c, nullptr));
     return hipSuccess;
 }
 
@@ -365,7 +367,6 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     }
     SA(salloc(s, &s->zero_bias, 128));
     SA(tensor(s, s->f0, S1, 24));
-    SA(tensor(s, s->a0, S1, 16));
     SA(tensor(s, s->e1, S1, 16));
     SA(tensor(s, s->p1, S2, 16));
     SA(tensor(s, s->a2, S2, 32));
@@ -387,7 +388,6 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     SA(tensor(s, s->u1, S1, 32));
     SA(tensor(s, s->c1, S1, 16));
     SA(tensor(s, s->x1, S1, 16));
-    SA(tensor(s, s->d1, S1, 16));
     SA(tensor(s, s->h, S1, 8));
     SA(tensor(s, s->g1a, S1, 16));
     SA(tensor(s, s->g1b, S1, 16));
@@ -484,9 +484,7 @@ int gell_surrogate_set_params(gell_surrogate* s, const float* host, int64_t n) {
 
 int gell_surrogate_predict(gell_surrogate* s, const float* field, const float* field1, const float* rate, float* out) {
     if (!s || !field || !field1 || !rate || !out) return sfail(s, GELL_E_ARG, "null argument");
-    hipError_t e = forward(s, field, field1, rate, false);
-    if (e == hipSuccess)
-        e = output_field(field, field1, s->h.d, s->N, s->norm, s->cfg.out_scale, out, s->stream);
+    hipError_t e = forward(s, field, field1, rate, false, false, out);
     return e == hipSuccess ? GELL_OK : sfail(s, GELL_E_HIP, std::string("predict: ") + hipGetErrorString(e));
 }
 
@@ -585,7 +583,10 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     if (e == hipSuccess) e = increment_norms(field, s->p_first, s->N, s->norm_part, s->norm, s->sums, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->h_sums, s->sums, sizeof(double) * 2, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipEventRecord(s->sums_ready, st);
-    if (e == hipSuccess && early) e = forward(s, field, s->p_first, rate, false, true);
+    // (a check's prediction goes to the scratch; a kept one over the field,
+    // which a fallback then rebuilds from p1 anyway)
+    float* const pred_to = check ? s->pred : field;
+    if (e == hipSuccess && early) e = forward(s, field, s->p_first, rate, false, true, pred_to);
     if (e == hipSuccess) e = hipEventSynchronize(s->sums_ready);
     if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance: ") + hipGetErrorString(e));
     const double rho = s->h_sums[1] > 0.0 ? std::sqrt(s->h_sums[0] / s->h_sums[1]) : 0.0;
@@ -599,8 +600,7 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     if (check) {
         // exact check: the prediction, then the other K - 1 real substeps,
         // kept; the prediction's error calibrates the estimate
-        e = output_field(field, s->p_first, s->h.d, s->N, s->norm, s->cfg.out_scale, s->pred, st);
-        if (e == hipSuccess) e = exact_rest();
+        e = exact_rest();
         double err = 0.0;
         if (e == hipSuccess) e = rel_change(s, field, s->pred, V, &err);
         if (e != hipSuccess) return sfail(s, GELL_E_HIP, std::string("advance check: ") + hipGetErrorString(e));
@@ -616,9 +616,8 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
         s->spent += est;
         *used = 1;
         ++s->n_accepted;
-        if (!early) e = forward(s, field, s->p_first, rate, false, true);
         // in place: the interior from p0 and the head, the outer layers from p1
-        if (e == hipSuccess) e = output_field(field, s->p_first, s->h.d, s->N, s->norm, s->cfg.out_scale, field, st);
+        if (!early) e = forward(s, field, s->p_first, rate, false, true, field);
     } else {
         ++s->n_fallback;
         e = exact_rest();

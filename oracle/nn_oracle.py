@@ -23,9 +23,9 @@ from __future__ import annotations
 import numpy as np
 
 LAYERS = [  # (cin, cout, level, relu)
-    (24, 16, 1, True), (16, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True),
-    (64, 64, 3, True), (64, 128, 4, True), (128, 128, 4, True), (128, 64, 3, True), (64, 64, 3, True),
-    (64, 32, 2, True), (32, 32, 2, True), (32, 16, 1, True), (16, 16, 1, True), (16, 8, 1, False),
+    (24, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True), (64, 64, 3, True),
+    (64, 128, 4, True), (128, 128, 4, True), (128, 64, 3, True), (64, 64, 3, True), (64, 32, 2, True),
+    (32, 32, 2, True), (32, 16, 1, True), (16, 8, 1, False),
 ]
 
 
@@ -168,30 +168,28 @@ def forward(params, p, p1, rate, cfg, rnd=ident, keep=False):
     c = lambda l, x: conv3(x, wr[l][0], wr[l][1], LAYERS[l][3], rnd)  # noqa: E731
     t = {}
     t["f0"] = features(p, p1, rate, cfg["rate_scale"], rnd)
-    t["a0"] = c(0, t["f0"])
-    t["e1"] = c(1, t["a0"])
+    t["e1"] = c(0, t["f0"])
     t["p1"] = pool(t["e1"], rnd)
-    t["a2"] = c(2, t["p1"])
-    t["e2"] = c(3, t["a2"])
+    t["a2"] = c(1, t["p1"])
+    t["e2"] = c(2, t["a2"])
     t["p2"] = pool(t["e2"], rnd)
-    t["a4"] = c(4, t["p2"])
-    t["e3"] = c(5, t["a4"])
+    t["a4"] = c(3, t["p2"])
+    t["e3"] = c(4, t["a4"])
     t["p3"] = pool(t["e3"], rnd)
-    t["a6"] = c(6, t["p3"])
-    t["e4"] = c(7, t["a6"])
+    t["a6"] = c(5, t["p3"])
+    t["e4"] = c(6, t["a6"])
     t["u3"] = up(t["e4"])
-    t["c3"] = c(8, t["u3"])
+    t["c3"] = c(7, t["u3"])
     t["x3"] = rnd(t["c3"] + t["e3"])
-    t["d3"] = c(9, t["x3"])
+    t["d3"] = c(8, t["x3"])
     t["u2"] = up(t["d3"])
-    t["c2"] = c(10, t["u2"])
+    t["c2"] = c(9, t["u2"])
     t["x2"] = rnd(t["c2"] + t["e2"])
-    t["d2"] = c(11, t["x2"])
+    t["d2"] = c(10, t["x2"])
     t["u1"] = up(t["d2"])
-    t["c1"] = c(12, t["u1"])
+    t["c1"] = c(11, t["u1"])
     t["x1"] = rnd(t["c1"] + t["e1"])
-    t["d1"] = c(13, t["x1"])
-    t["h"] = c(14, t["d1"])
+    t["h"] = c(12, t["x1"])
     s, _ = increment_scale(p, p1)
     out = p + s * cfg["out_scale"] * head_to_fine(t["h"])
     out = np.where(interior(p.shape[0]), out, p1)
@@ -227,36 +225,33 @@ def loss_and_grad(params, p, p1, rate, target, cfg, rnd=ident):
         grads[l] = (dW, db)
         return dx
 
-    g = back(14, t["d1"], dh)
-    g = relu_mask(g, t["d1"])
-    dx1 = back(13, t["x1"], g)
+    dx1 = back(12, t["x1"], dh)
     de1 = dx1
     g = relu_mask(dx1, t["c1"])
-    du1 = back(12, t["u1"], g)
+    du1 = back(11, t["u1"], g)
     g = relu_mask(up_back(du1, rnd), t["d2"])
-    dx2 = back(11, t["x2"], g)
+    dx2 = back(10, t["x2"], g)
     de2 = dx2
     g = relu_mask(dx2, t["c2"])
-    du2 = back(10, t["u2"], g)
+    du2 = back(9, t["u2"], g)
     g = relu_mask(up_back(du2, rnd), t["d3"])
-    dx3 = back(9, t["x3"], g)
+    dx3 = back(8, t["x3"], g)
     de3 = dx3
     g = relu_mask(dx3, t["c3"])
-    du3 = back(8, t["u3"], g)
+    du3 = back(7, t["u3"], g)
     g = relu_mask(up_back(du3, rnd), t["e4"])
-    g = relu_mask(back(7, t["a6"], g), t["a6"])
-    dp3 = back(6, t["p3"], g)
+    g = relu_mask(back(6, t["a6"], g), t["a6"])
+    dp3 = back(5, t["p3"], g)
     de3 = rnd(pool_back(dp3, rnd) + de3)
     g = relu_mask(de3, t["e3"])
-    g = relu_mask(back(5, t["a4"], g), t["a4"])
-    dp2 = back(4, t["p2"], g)
+    g = relu_mask(back(4, t["a4"], g), t["a4"])
+    dp2 = back(3, t["p2"], g)
     de2 = rnd(pool_back(dp2, rnd) + de2)
     g = relu_mask(de2, t["e2"])
-    g = relu_mask(back(3, t["a2"], g), t["a2"])
-    dp1 = back(2, t["p1"], g)
+    g = relu_mask(back(2, t["a2"], g), t["a2"])
+    dp1 = back(1, t["p1"], g)
     de1 = rnd(pool_back(dp1, rnd) + de1)
     g = relu_mask(de1, t["e1"])
-    g = relu_mask(back(1, t["a0"], g), t["a0"])
     back(0, t["f0"], g, need_dx=False)
     w_off, b_off, total = param_layout()
     flat = np.zeros(total)

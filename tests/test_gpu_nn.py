@@ -4,7 +4,7 @@ each 3x3x3 layer shape (forward, input gradient, weight gradient), then the
 whole network's prediction, loss and parameter gradients, then a few Adam
 steps. Tolerances: one BF16 rounding step (2^-8 relative) on BF16 outputs,
 1e-5 on FP32-accumulated weight gradients of exact BF16 products, and
-5e-2 relative L2 on whole-network gradients (15 layers of BF16 activations
+5e-2 relative L2 on whole-network gradients (13 layers of BF16 activations
 and gradients)."""
 import numpy as np
 import pytest
