@@ -154,9 +154,21 @@ inline int32_t grid_count(const gell_ctx* c, int32_t n) { return c->devn ? c->P.
 
 // Each launcher enqueues on ctx->stream and returns a hipError_t.
 hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st);
+// Increment norms of an out-of-place substep, summed by its last kernel:
+// per-block partials (sum over the interior of (dst - p0)^2, sum of p0^2)
+// into part; nblk = how many blocks wrote them (0: this substep form does
+// not sum them, and the caller must).
+struct SubstepNorms {
+    const float* p0 = nullptr;
+    double* part = nullptr;
+    int nblk = 0;
+};
 // One substep from the field in src into dst (both N^3; src == dst: in place;
 // src may be the context's field and dst not, and the other way round).
-hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st);
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st,
+                              SubstepNorms* norms = nullptr);
+// Largest nblk lod_substep_launch reports for grid side N.
+inline int substep_norm_blocks(int N) { return N * N / 64; }
 hipError_t xspan_rebuild(gell_ctx* ctx, int s, hipStream_t st);
 // Make the context stream wait for substrate s's own stream (no-op when
 // nothing is pending there); join_substrates does it for every substrate.

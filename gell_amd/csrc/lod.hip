@@ -750,10 +750,17 @@ __global__ __launch_bounds__(256) void lod_y_reduce(const float* __restrict__ ta
     }
 }
 
-template <int N, int PZ>
+// With NORMS the kernel also sums, per block in a fixed order, (p - p0)^2
+// over the interior (voxels two or more from every face) and p0^2 over all
+// voxels, in double, into part[2 block], part[2 block + 1]: the increment
+// norms of a first substep run out of place (p0 its input), as incr_sums
+// would compute them from the two fields, without reading p again.
+template <int N, int PZ, bool NORMS = false>
 __global__ __launch_bounds__(64 * PZ) void lod_z_part_ycorr(float* __restrict__ p, const float* __restrict__ tabz,
                                                             const float* __restrict__ taby,
-                                                            const float* __restrict__ coup, float side, float bnd) {
+                                                            const float* __restrict__ coup, float side, float bnd,
+                                                            const float* __restrict__ p0 = nullptr,
+                                                            double* __restrict__ part = nullptr) {
     constexpr int PY = N / 64;
     constexpr int64_t NN = (int64_t)N * N;
     constexpr int m = N / PZ;
@@ -792,6 +799,38 @@ __global__ __launch_bounds__(64 * PZ) void lod_z_part_ycorr(float* __restrict__ 
 #pragma unroll
     for (int t = 0; t < m; ++t)
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v[t]), rs, voff, so0 + t * stride_b, 0);
+    if constexpr (NORMS) {
+        const __amdgpu_buffer_rsrc_t r0 = field_rsrc(p0, NN * N);
+        const bool line_in = i >= 2 && i < N - 2 && j >= 2 && j < N - 2;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int t = 0; t < m; ++t) {
+            const int kz = kb * m + t;
+            const double a =
+                (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r0, voff, so0 + t * stride_b, 0));
+            s1 += a * a;
+            if (line_in && kz >= 2 && kz < N - 2) {
+                const double d = (double)v[t] - a;
+                s0 += d * d;
+            }
+        }
+        // block sums in a fixed order: the waves' lanes by tree in LDS
+        __shared__ double r0s[64 * PZ], r1s[64 * PZ];
+        r0s[threadIdx.x] = s0;
+        r1s[threadIdx.x] = s1;
+        __syncthreads();
+        for (int w = 32 * PZ; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                r0s[threadIdx.x] += r0s[threadIdx.x + w];
+                r1s[threadIdx.x] += r1s[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = r0s[0];
+            part[2 * blockIdx.x + 1] = r1s[0];
+        }
+    }
 }
 
 // Any N, any direction (dir 0 = x with fused boundary + uptake).
@@ -964,16 +1003,24 @@ hipError_t launch_mixed(float* p, const float* rate, const int2* xspan, const fl
 
 template <int N, int PZ>
 void launch_z_ycorr(float* p, const float* tabz, const float* taby, const float* coup, float side, float bnd,
-                    hipStream_t st) {
-    hipLaunchKernelGGL((lod_z_part_ycorr<N, PZ>), dim3((unsigned)((int64_t)N * N / 64)), dim3(64 * PZ), 0, st, p,
-                       tabz, taby, coup, side, bnd);
+                    hipStream_t st, SubstepNorms* norms) {
+    const dim3 grid((unsigned)((int64_t)N * N / 64));
+    if (norms) {
+        hipLaunchKernelGGL((lod_z_part_ycorr<N, PZ, true>), grid, dim3(64 * PZ), 0, st, p, tabz, taby, coup, side, bnd,
+                           norms->p0, norms->part);
+        norms->nblk = (int)grid.x;
+    } else {
+        hipLaunchKernelGGL((lod_z_part_ycorr<N, PZ>), grid, dim3(64 * PZ), 0, st, p, tabz, taby, coup, side, bnd,
+                           nullptr, nullptr);
+    }
 }
 
 // Fused substep (see lod_xy_part); zp = z blocks, falling back to the next
 // smaller count the grid supports.
 template <int N>
 hipError_t launch_fused(const float* src, float* p, const float* rate, const int2* xspan, float* const* tabs,
-                        float side, float bnd, float dt, int zp, float* coup, hipStream_t st) {
+                        float side, float bnd, float dt, int zp, float* coup, hipStream_t st,
+                        SubstepNorms* norms = nullptr) {
     constexpr int P = N / 64;
     static_assert(P == 2 || P == 4 || P == 8, "fused substep needs N in {128, 256, 512}");
     const float* taby = tabs[P == 2 ? 0 : (P == 4 ? 1 : 2)];
@@ -985,13 +1032,13 @@ hipError_t launch_fused(const float* src, float* p, const float* rate, const int
                        coup);
     constexpr bool ok4 = N % (4 * kChunk) == 0, ok8 = N % (8 * kChunk) == 0, ok16 = N % (16 * kChunk) == 0;
     if (zp >= 16 && ok16)
-        launch_z_ycorr<N, ok16 ? 16 : 2>(p, tabs[3], taby, coup, side, bnd, st);
+        launch_z_ycorr<N, ok16 ? 16 : 2>(p, tabs[3], taby, coup, side, bnd, st, norms);
     else if (zp >= 8 && ok8)
-        launch_z_ycorr<N, ok8 ? 8 : 2>(p, tabs[2], taby, coup, side, bnd, st);
+        launch_z_ycorr<N, ok8 ? 8 : 2>(p, tabs[2], taby, coup, side, bnd, st, norms);
     else if (zp >= 4 && ok4)
-        launch_z_ycorr<N, ok4 ? 4 : 2>(p, tabs[1], taby, coup, side, bnd, st);
+        launch_z_ycorr<N, ok4 ? 4 : 2>(p, tabs[1], taby, coup, side, bnd, st, norms);
     else
-        launch_z_ycorr<N, 2>(p, tabs[0], taby, coup, side, bnd, st);
+        launch_z_ycorr<N, 2>(p, tabs[0], taby, coup, side, bnd, st, norms);
     return hipGetLastError();
 }
 
@@ -1010,9 +1057,11 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, hipStream_t st) {
 // One substep from the field in src to the field in dst (src == dst: in
 // place). The fused substep reads src in its first kernel; the other forms
 // copy src to dst first and work in place on dst.
-hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st) {
+hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst, hipStream_t st,
+                              SubstepNorms* norms) {
     const int N = ctx->N;
     float* p = dst;
+    if (norms) norms->nblk = 0;
     const float* rate = ctx->rate[s];
     const float* E = ctx->E[s];
     const float* F = ctx->F[s];
@@ -1030,9 +1079,9 @@ hipError_t lod_substep_launch(gell_ctx* ctx, int s, const float* src, float* dst
     }
     if (lod_fused_active(ctx)) {
         float* const* tabs = ctx->ptab[s];
-        if (N == 128) return launch_fused<128>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        if (N == 256) return launch_fused<256>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
-        return launch_fused<512>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st);
+        if (N == 128) return launch_fused<128>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st, norms);
+        if (N == 256) return launch_fused<256>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st, norms);
+        return launch_fused<512>(src, p, rate, xspan, tabs, side, bnd, dt, yzp, ctx->coup[s], st, norms);
     }
     if (src != p) {
         const hipError_t e = hipMemcpyAsync(p, src, sizeof(float) * (size_t)ctx->V, hipMemcpyDeviceToDevice, st);

@@ -90,17 +90,6 @@ __device__ __forceinline__ void stage_halo(const uint16_t* base, int X, int Y, i
 __device__ __forceinline__ void stage_wait() { __builtin_amdgcn_s_waitcnt(0); }
 constexpr int kTX = 8, kTY = 8;
 
-// The network's output written by its last layer (the head) instead of a
-// BF16 head tensor: out at the fine voxels of each coarse voxel, as head_out
-// forms it (see there); `out` may alias p0 or p1.
-struct HeadOut {
-    const float* p0;
-    const float* p1;
-    const float* norm;
-    float out_scale;
-    float* out;
-};
-
 // ---------------------------------------------------------------------------
 // Forward convolution. Block = 8 x 8 x TZ output voxels and CG of the layer's
 // 16-channel output tiles (blockIdx.z also walks the NCT / CG tile groups);
@@ -113,10 +102,10 @@ struct HeadOut {
 // loads per wave.
 // Epilogue: + bias, ReLU (when relu), BF16 store.
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int TZ, int CG, bool HEAD = false>
+template <int CIN, int COUT, int TZ, int CG>
 __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                       const float* __restrict__ bias, uint16_t* __restrict__ y, int X,
-                                                      int Y, int Z, int relu, HeadOut ho = HeadOut{}) {
+                                                      int Y, int Z, int relu, uint16_t* __restrict__ pooled) {
     constexpr int HX = kTX + 2, HY = kTY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int CH = CIN / 8;
     constexpr int KP = kpad(CIN);
@@ -198,6 +187,7 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
             for (int c = 0; c < CG; ++c)
                 acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c], bf[a], acc[a][c], 0, 0, 0);
     }
+    uint2 hv[VPW][CG];  // the stored BF16 values, kept for the pooling below
 #pragma unroll
     for (int a = 0; a < VPW; ++a) {
         const int v = (wv + 4 * a) * 16 + r;
@@ -207,6 +197,7 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
 #pragma unroll
         for (int c = 0; c < CG; ++c) {
             const int o = (ct0 + c) * 16 + 4 * g;
+            hv[a][c] = make_uint2(0u, 0u);
             if (o >= COUT) continue;
             float t[4];
 #pragma unroll
@@ -214,31 +205,54 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd(const uint16_t* __restrict
                 t[i] = acc[a][c][i] + bias[o + i];
                 if (relu) t[i] = fmaxf(t[i], 0.f);
             }
-            const uint2 pk = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
-            if constexpr (HEAD) {
-                // head channels o .. o + 3 are fine voxels sub = o + i of
-                // coarse voxel (gx, gy, gz): z parity o >> 2, (y, x) parity
-                // (i >> 1, i & 1); the BF16-rounded head value, as head_out
-                // reads it from the head tensor
-                const int N = 2 * X;
-                const float scale = ho.norm[0] * ho.out_scale;
-                const float h[4] = {__uint_as_float(pk.x << 16), __uint_as_float(pk.x & 0xffff0000u),
-                                    __uint_as_float(pk.y << 16), __uint_as_float(pk.y & 0xffff0000u)};
-                const int zi = 2 * gz + (o >> 2);
+            hv[a][c] = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
+            *reinterpret_cast<uint2*>(y + vox * COUT + o) = hv[a][c];
+        }
+    }
+    if constexpr (VPW % 2 == 0) {
+        if (pooled) {
+            // pool2 of the output, from registers: the 2 x 2 x 2 cell of an
+            // even voxel is lanes r, r ^ 1 (x), r ^ 8 (y), r ^ 9 of tiles a
+            // and a + 1 (z); summed in pool2's order (x fastest, then y, z).
+            // The lane exchanges stay inside 16-lane rows: DPP moves (a
+            // quad permutation for r ^ 1, a rotation by 8 for r ^ 8), not
+            // LDS permutes, which this kernel's B reads keep busy.
+            const auto xor1 = [](float v) {
+                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF,
+                                                                             0xF, false));
+            };
+            const auto xor8 = [](float v) {
+                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF,
+                                                                             0xF, false));
+            };
+            const bool writer = (r & 1) == 0 && r < 8;
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const int yi = 2 * gy + q, x0 = 2 * gx, x1 = 2 * gx + 1;
-                    const int64_t fo = ((int64_t)zi * N + yi) * N + x0;
-                    const float2 pa = *reinterpret_cast<const float2*>(ho.p0 + fo);
-                    const float2 pb = *reinterpret_cast<const float2*>(ho.p1 + fo);
-                    const bool row_edge = yi < 2 || zi < 2 || yi >= N - 2 || zi >= N - 2;
-                    float2 rv;
-                    rv.x = (row_edge || x0 < 2 || x0 >= N - 2) ? pb.x : pa.x + scale * h[2 * q];
-                    rv.y = (row_edge || x1 < 2 || x1 >= N - 2) ? pb.y : pa.y + scale * h[2 * q + 1];
-                    *reinterpret_cast<float2*>(ho.out + fo) = rv;
+            for (int a = 0; a < VPW; a += 2) {
+#pragma unroll
+                for (int c = 0; c < CG; ++c) {
+                    const int o = (ct0 + c) * 16 + 4 * g;
+                    float q[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t w0 = i < 2 ? hv[a][c].x : hv[a][c].y, w1 = i < 2 ? hv[a + 1][c].x : hv[a + 1][c].y;
+                        const float z0 = __uint_as_float((i & 1) ? (w0 & 0xffff0000u) : (w0 << 16));
+                        const float z1 = __uint_as_float((i & 1) ? (w1 & 0xffff0000u) : (w1 << 16));
+                        const float c0[4] = {z0, xor1(z0), xor8(z0), xor8(xor1(z0))};
+                        const float c1[4] = {z1, xor1(z1), xor8(z1), xor8(xor1(z1))};
+                        float sum = 0.f;
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) sum += c0[d];
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) sum += c1[d];
+                        q[i] = sum * 0.125f;
+                    }
+                    const int v = (wv + 4 * a) * 16 + r;
+                    const int px = (blockIdx.x * kTX + v % kTX) / 2, py = (blockIdx.y * kTY + (v / kTX) % kTY) / 2,
+                              pz = (bz * TZ + v / (kTX * kTY)) / 2;
+                    if (writer && o < COUT)
+                        *reinterpret_cast<uint2*>(pooled + (((int64_t)pz * (Y / 2) + py) * (X / 2) + px) * COUT + o) =
+                            make_uint2(pack_bf2(q[0], q[1]), pack_bf2(q[2], q[3]));
                 }
-            } else {
-                *reinterpret_cast<uint2*>(y + vox * COUT + o) = pk;
             }
         }
     }
@@ -805,7 +819,8 @@ __global__ __launch_bounds__(256) void incr_sums(const float* __restrict__ p0, c
 // norm[1] = 1 / s (0 when s = 0: then the prediction is p0 itself);
 // sums[0..1] = the two sums.
 __global__ __launch_bounds__(256) void incr_finalize(const double* __restrict__ part, int nblk, double inv_count,
-                                                     float* __restrict__ norm, double* __restrict__ sums) {
+                                                     float* __restrict__ norm, double* __restrict__ sums,
+                                                     double* host_sums) {
     __shared__ double r0[256], r1[256];
     double s0 = 0.0, s1 = 0.0;
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -828,6 +843,12 @@ __global__ __launch_bounds__(256) void incr_finalize(const double* __restrict__ 
         norm[1] = s > 0.0 ? (float)(1.0 / s) : 0.f;
         sums[0] = r0[0];
         sums[1] = r1[0];
+        if (host_sums) {
+            // the two sums straight into pinned host memory as well (the
+            // host reads them after an event, without a copy in the stream)
+            host_sums[0] = r0[0];
+            host_sums[1] = r1[0];
+        }
     }
 }
 
@@ -895,13 +916,15 @@ __global__ void head_out(const float* p0, const float* p1, const uint16_t* __res
     for (int q = 0; q < 4; ++q) {
         const int yi = 2 * y + (q & 1), zi = 2 * z + (q >> 1);
         const int64_t o = ((int64_t)zi * N + yi) * N + 2 * x;
-        const float2 a = *reinterpret_cast<const float2*>(p0 + o);
-        const float2 b = *reinterpret_cast<const float2*>(p1 + o);
         const bool row_edge = yi < 2 || zi < 2 || yi >= N - 2 || zi >= N - 2;
         const int x0 = 2 * x, x1 = 2 * x + 1;
+        const bool e0 = row_edge || x0 < 2 || x0 >= N - 2, e1 = row_edge || x1 < 2 || x1 >= N - 2;
+        // p1 only on the outer layers, p0 only inside: each read where used
+        const float2 a = (e0 && e1) ? make_float2(0.f, 0.f) : *reinterpret_cast<const float2*>(p0 + o);
+        const float2 b = (e0 || e1) ? *reinterpret_cast<const float2*>(p1 + o) : make_float2(0.f, 0.f);
         float2 r;
-        r.x = (row_edge || x0 < 2 || x0 >= N - 2) ? b.x : a.x + scale * h[q * 2];
-        r.y = (row_edge || x1 < 2 || x1 >= N - 2) ? b.y : a.y + scale * h[q * 2 + 1];
+        r.x = e0 ? b.x : a.x + scale * h[q * 2];
+        r.y = e1 ? b.y : a.y + scale * h[q * 2 + 1];
         *reinterpret_cast<float2*>(out + o) = r;
     }
 }
@@ -977,9 +1000,10 @@ constexpr int tz_for(int cin) { return cin <= 16 ? 8 : (cin <= 32 ? 4 : (cin == 
 
 template <int CIN, int COUT>
 hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int X, int Y, int Z, int relu,
-                  hipStream_t st) {
+                  hipStream_t st, uint16_t* pooled, bool* pooled_done) {
     constexpr int TZ = tz_for(CIN);
     constexpr int NCT = cpad(COUT) / 16;
+    if (pooled_done) *pooled_done = false;
     if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
     if constexpr (CIN % 32 == 0) {
         // coarse levels (grids up to 32 per side): K split over the waves
@@ -991,12 +1015,15 @@ hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t
         }
     }
     const int64_t base = (int64_t)(X / kTX) * (Y / kTY) * (Z / TZ);
+    // blocks of two or more z slices can pool their own output
+    uint16_t* const pool_to = TZ % 2 == 0 ? pooled : nullptr;
     if (base >= 256 || NCT == 1)
         hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ, NCT>), dim3(X / kTX, Y / kTY, Z / TZ), dim3(kThreads), 0, st, x,
-                           w, b, y, X, Y, Z, relu);
+                           w, b, y, X, Y, Z, relu, pool_to);
     else
         hipLaunchKernelGGL((conv3_fwd<CIN, COUT, TZ, 1>), dim3(X / kTX, Y / kTY, (Z / TZ) * NCT), dim3(kThreads), 0,
-                           st, x, w, b, y, X, Y, Z, relu);
+                           st, x, w, b, y, X, Y, Z, relu, pool_to);
+    if (pooled_done) *pooled_done = pool_to != nullptr;
     return hipGetLastError();
 }
 
@@ -1040,9 +1067,9 @@ int64_t wgrad_scratch_floats(int cin, int cout, int X, int Y, int Z) {
 }
 
 hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int cin, int cout, int X, int Y,
-                 int Z, int relu, hipStream_t st) {
+                 int Z, int relu, hipStream_t st, uint16_t* pooled, bool* pooled_done) {
 #define GELL_NN_CASE(a, c) \
-    if (cin == a && cout == c) return conv_t<a, c>(x, w, b, y, X, Y, Z, relu, st);
+    if (cin == a && cout == c) return conv_t<a, c>(x, w, b, y, X, Y, Z, relu, st, pooled, pooled_done);
     GELL_NN_CASE(24, 16) GELL_NN_CASE(16, 24) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32)
     GELL_NN_CASE(32, 64) GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64)
     GELL_NN_CASE(64, 32) GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8) GELL_NN_CASE(8, 16)
@@ -1132,11 +1159,19 @@ hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, h
 }
 
 hipError_t increment_norms(const float* p0, const float* p1, int N, double* part, float* norm, double* sums,
-                           hipStream_t st) {
+                           hipStream_t st, double* host_sums) {
     const int64_t interior = (int64_t)(N - 4) * (N - 4) * (N - 4);
     hipLaunchKernelGGL(incr_sums, dim3(kNormBlocks), dim3(256), 0, st, p0, p1, N, part);
     hipLaunchKernelGGL(incr_finalize, dim3(1), dim3(256), 0, st, part, kNormBlocks, 1.0 / (double)interior, norm,
-                       sums);
+                       sums, host_sums);
+    return hipGetLastError();
+}
+
+hipError_t finalize_norms(const double* part, int nblk, int N, float* norm, double* sums, hipStream_t st,
+                          double* host_sums) {
+    const int64_t interior = (int64_t)(N - 4) * (N - 4) * (N - 4);
+    hipLaunchKernelGGL(incr_finalize, dim3(1), dim3(256), 0, st, part, nblk, 1.0 / (double)interior, norm, sums,
+                       host_sums);
     return hipGetLastError();
 }
 
@@ -1151,17 +1186,6 @@ hipError_t output_field(const float* p0, const float* p1, const uint16_t* head, 
                         float out_scale, float* out, hipStream_t st) {
     hipLaunchKernelGGL(head_out, dim3(blocks_for((int64_t)(N / 2) * (N / 2) * (N / 2))), dim3(256), 0, st, p0, p1,
                        head, N, norm, out_scale, out);
-    return hipGetLastError();
-}
-
-hipError_t conv3_head_out(const uint16_t* x, const uint16_t* w, const float* b, const float* p0, const float* p1,
-                          int N, const float* norm, float out_scale, float* out, hipStream_t st) {
-    constexpr int TZ = tz_for(16);
-    const int X = N / 2;
-    if (X % kTX || X % TZ) return hipErrorInvalidValue;
-    const HeadOut ho{p0, p1, norm, out_scale, out};
-    hipLaunchKernelGGL((conv3_fwd<16, 8, TZ, 1, true>), dim3(X / kTX, X / kTY, X / TZ), dim3(kThreads), 0, st, x, w, b,
-                       nullptr, X, X, X, 0, ho);
     return hipGetLastError();
 }
 

@@ -16,8 +16,10 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 constexpr int cpad(int c) { return (c + 15) / 16 * 16; }
 
 // y = act(conv3x3x3(x, w) + b); w in GEMM layout [cpad(cout)][kpad(cin)].
+// With pooled, the kernel also writes pool2(y) there when its blocks can
+// (*pooled_done then true; otherwise the caller pools y itself).
 hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, int cin, int cout, int X, int Y,
-                 int Z, int relu, hipStream_t st);
+                 int Z, int relu, hipStream_t st, uint16_t* pooled = nullptr, bool* pooled_done = nullptr);
 // y = act(conv3x3x3(up2(d), W) + b) (+ skip) without forming up2(d): d is
 // S^3 x cin, y (2S)^3 x cout; w8 = up_weights of the layer's FP32 master
 // weights ([8][cpad(cout)][8 cin], up_weight_count elements). (cin, cout) in
@@ -52,8 +54,14 @@ hipError_t add(const uint16_t* a, const uint16_t* b, uint16_t* out, int64_t n, h
 // (0 when s = 0), and sums[0] = sum of r^2 over the interior, sums[1] = sum
 // of p0^2 over the grid.
 constexpr int kNormBlocks = 1024;
+// host_sums (optional): a device-mapped pointer to pinned host memory that
+// receives the two sums as well.
 hipError_t increment_norms(const float* p0, const float* p1, int N, double* part, float* norm, double* sums,
-                           hipStream_t st);
+                           hipStream_t st, double* host_sums = nullptr);
+// The same reduction from nblk partials summed elsewhere (an out-of-place
+// substep's last kernel).
+hipError_t finalize_norms(const double* part, int nblk, int N, float* norm, double* sums, hipStream_t st,
+                          double* host_sums = nullptr);
 // p0, p1 (N^3) + uptake map -> 24-channel features on the (N/2)^3 grid: r / s
 // as BF16 high and low parts, rate * rate_scale (8 fine voxels each)
 hipError_t input_features(const float* p0, const float* p1, const float* rate, int N, const float* norm,
@@ -62,12 +70,6 @@ hipError_t input_features(const float* p0, const float* p1, const float* rate, i
 // the two outer layers (sub-voxel order z, y, x); out may alias p1
 hipError_t output_field(const float* p0, const float* p1, const uint16_t* head, int N, const float* norm,
                         float out_scale, float* out, hipStream_t st);
-// The head layer (16 -> 8 channels on the (N/2)^3 grid of x, no ReLU) with
-// output_field fused into it: out (N^3) from p0, p1 and the BF16-rounded
-// head values, bit for bit what conv3 into a head tensor + output_field give;
-// out may alias p0 or p1.
-hipError_t conv3_head_out(const uint16_t* x, const uint16_t* w, const float* b, const float* p0, const float* p1,
-                          int N, const float* norm, float out_scale, float* out, hipStream_t st);
 // mean squared error of the increment in units of s out_scale over the
 // interior; dL/dhead and per-block loss partial sums (*nparts of them)
 hipError_t loss_head(const float* p0, const uint16_t* head, const float* target, int N, const float* norm,

@@ -34,6 +34,7 @@
 // Every layer but the head applies ReLU; the "+ e" skips are added after it.
 // Training (gradients + Adam) runs on the same kernels: input gradients are
 // convolutions with the flipped weights, weight gradients MFMA reductions.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -95,8 +96,9 @@ struct gell_surrogate {
     double* norm_part = nullptr;  // increment_norms partials
     float* norm = nullptr;        // s, 1 / s (device)
     double* sums = nullptr;       // sum r^2 (interior), sum p0^2 (device)
-    double* h_sums = nullptr;     // pinned copy of sums
-    hipEvent_t sums_ready = nullptr;  // recorded after the copy of sums to h_sums
+    double* h_sums = nullptr;     // pinned, coherent copy of sums
+    double* h_sums_dev = nullptr; // its device-mapped address (written by the norms' last kernel)
+    hipEvent_t sums_ready = nullptr;  // recorded after the kernel that writes h_sums
     int adam_t = 0;
     int accumulate = 0;  // add this call's gradients to the stored ones
     // The in-loop gate (gell_surrogate_advance): the largest measured ratio
@@ -162,6 +164,17 @@ hipError_t conv_layer(gell_surrogate* s, int l, const Tensor& in, Tensor& out) {
                  s->stream);
 }
 
+// Layer l, then pooled = pool2(out): in the convolution's own epilogue where
+// its blocks allow, else by the pooling kernel (the same values either way).
+hipError_t conv_pool_layer(gell_surrogate* s, int l, const Tensor& in, Tensor& out, Tensor& pooled) {
+    const LayerSpec& L = kLayers[l];
+    bool done = false;
+    SCHK(conv3(in.d, s->wb[l], s->param + s->b_off[l], out.d, L.cin, L.cout, in.S, in.S, in.S, L.relu ? 1 : 0,
+               s->stream, pooled.d, &done));
+    if (!done) SCHK(pool(out.d, pooled.d, pooled.S, pooled.S, pooled.S, pooled.C, s->stream));
+    return hipSuccess;
+}
+
 // out = layer l (one of 8, 10, 12) of up2(coarse) + skip, on the coarse grid.
 hipError_t up_layer(gell_surrogate* s, int l, const Tensor& coarse, Tensor& out, const uint16_t* skip) {
     const LayerSpec& L = kLayers[l];
@@ -178,22 +191,18 @@ hipError_t up_layer(gell_surrogate* s, int l, const Tensor& coarse, Tensor& out,
 // where the formed path rounds each tap, a difference of one BF16 rounding
 // per weight) and store the sum themselves.
 // norms_ready: the norms of this (p0, p1) are already on the device. With
-// out (inference only) the head layer writes the predicted field there
-// itself (conv3_head_out); otherwise it writes the head tensor h.
+// out (inference only) the predicted field is written there as well.
 hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const float* rate, bool train,
                    bool norms_ready = false, float* out = nullptr) {
     if (!s->packed) SCHK(pack_all(s));
     hipStream_t st = s->stream;
     if (!norms_ready) SCHK(increment_norms(p0, p1, s->N, s->norm_part, s->norm, s->sums, st));
     SCHK(input_features(p0, p1, rate, s->N, s->norm, s->cfg.rate_scale, s->f0.d, st));
-    SCHK(conv_layer(s, 0, s->f0, s->e1));
-    SCHK(pool(s->e1.d, s->p1.d, s->p1.S, s->p1.S, s->p1.S, s->p1.C, st));
+    SCHK(conv_pool_layer(s, 0, s->f0, s->e1, s->p1));
     SCHK(conv_layer(s, 1, s->p1, s->a2));
-    SCHK(conv_layer(s, 2, s->a2, s->e2));
-    SCHK(pool(s->e2.d, s->p2.d, s->p2.S, s->p2.S, s->p2.S, s->p2.C, st));
+    SCHK(conv_pool_layer(s, 2, s->a2, s->e2, s->p2));
     SCHK(conv_layer(s, 3, s->p2, s->a4));
-    SCHK(conv_layer(s, 4, s->a4, s->e3));
-    SCHK(pool(s->e3.d, s->p3.d, s->p3.S, s->p3.S, s->p3.S, s->p3.C, st));
+    SCHK(conv_pool_layer(s, 4, s->a4, s->e3, s->p3));
     SCHK(conv_layer(s, 5, s->p3, s->a6));
     SCHK(conv_layer(s, 6, s->a6, s->e4));
     if (train) {
@@ -219,11 +228,8 @@ hipError_t forward(gell_surrogate* s, const float* p0, const float* p1, const fl
     } else {
         SCHK(up_layer(s, 11, s->d2, s->x1, s->e1.d));
     }
-    if (out && !train)
-        SCHK(conv3_head_out(s->x1.d, s->wb[12], s->param + s->b_off[12], p0, p1, s->N, s->norm, s->cfg.out_scale,
-                            out, st));
-    else
-        SCHK(conv_layer(s, 12, s->x1, s->h));
+    SCHK(conv_layer(s, 12, s->x1, s->h));
+    if (out && !train) SCHK(output_field(p0, p1, s->h.d, s->N, s->norm, s->cfg.out_scale, out, st));
     return hipSuccess;
 }
 
@@ -416,11 +422,14 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
     SA(salloc(s, &s->pred, (int64_t)s->N * s->N * s->N));
     SA(salloc(s, &s->p_first, (int64_t)s->N * s->N * s->N));
     SA(salloc(s, &s->red_part, 2 * kRedBlocks));
-    SA(salloc(s, &s->norm_part, 2 * kNormBlocks));
+    SA(salloc(s, &s->norm_part, 2 * std::max(kNormBlocks, gell::substep_norm_blocks(s->N))));
     SA(salloc(s, &s->norm, 2));
     SA(salloc(s, &s->sums, 2));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_red), sizeof(float) * (2 * kRedBlocks + 8));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_sums), sizeof(double) * 2);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&s->h_sums), sizeof(double) * 2,
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->h_sums_dev), s->h_sums, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sums_ready, hipEventDisableTiming);
 #undef SA
     if (e == hipSuccess) {
@@ -579,9 +588,14 @@ int gell_surrogate_advance(gell_surrogate* s, gell_ctx* ctx, int32_t sub, float 
     // goes to the scratch; then its increment's norms, read back while the
     // network runs
     hipError_t e = gell::join_substrates(ctx);
-    if (e == hipSuccess) e = gell::lod_substep_launch(ctx, sub, field, s->p_first, st);
-    if (e == hipSuccess) e = increment_norms(field, s->p_first, s->N, s->norm_part, s->norm, s->sums, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->h_sums, s->sums, sizeof(double) * 2, hipMemcpyDeviceToHost, st);
+    gell::SubstepNorms sn;
+    sn.p0 = field;
+    sn.part = s->norm_part;
+    if (e == hipSuccess) e = gell::lod_substep_launch(ctx, sub, field, s->p_first, st, &sn);
+    // the fused substep sums the norms in its z sweep; the other forms do not
+    if (e == hipSuccess)
+        e = sn.nblk ? finalize_norms(sn.part, sn.nblk, s->N, s->norm, s->sums, st, s->h_sums_dev)
+                    : increment_norms(field, s->p_first, s->N, s->norm_part, s->norm, s->sums, st, s->h_sums_dev);
     if (e == hipSuccess) e = hipEventRecord(s->sums_ready, st);
     // (a check's prediction goes to the scratch; a kept one over the field,
     // which a fallback then rebuilds from p1 anyway)
@@ -669,8 +683,9 @@ int gell_nn_upconv3(const uint16_t* x, const float* w_master, const float* b, co
 // Single 3x3x3 layer on an S^3 grid, for checking the kernels one at a time.
 // x, y BF16 device tensors; w_master FP32 [cout][cin][27] and b FP32 on the
 // host. Allocates and frees its own packing buffers; synchronises.
-int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
-                  int32_t S, int32_t relu, int32_t input_grad) {
+namespace {
+int nn_conv3_impl(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
+                  int32_t S, int32_t relu, int32_t input_grad, uint16_t* pooled, int32_t* in_kernel) {
     if (!x || !w_master || !b || !y || S < 8 || S % 8) return sfail(nullptr, GELL_E_ARG, "bad conv arguments");
     const int64_t nw = (int64_t)cout * cin * 27;
     float *dm = nullptr, *db = nullptr;
@@ -685,13 +700,28 @@ int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint
     if (e == hipSuccess) e = weights_pack(dm, cin, cout, wb, nullptr);
     if (e == hipSuccess && input_grad) e = weights_flip(wb, cin, cout, wf, nullptr);
     // input_grad: y (cin channels) = input gradient of the layer for x = dy (cout channels)
+    bool done = false;
     if (e == hipSuccess)
         e = input_grad ? conv3(x, wf, db, y, cout, cin, S, S, S, 0, nullptr)
-                       : conv3(x, wb, db, y, cin, cout, S, S, S, relu, nullptr);
+                       : conv3(x, wb, db, y, cin, cout, S, S, S, relu, nullptr, pooled, &done);
+    if (e == hipSuccess && pooled && !done) e = pool(y, pooled, S / 2, S / 2, S / 2, cout, nullptr);
+    if (in_kernel) *in_kernel = done ? 1 : 0;
     if (e == hipSuccess) e = hipDeviceSynchronize();
     for (void* p : {(void*)dm, (void*)db, (void*)wb, (void*)wf})
         if (p) (void)hipFree(p);
     return e == hipSuccess ? GELL_OK : sfail(nullptr, GELL_E_HIP, std::string("conv3: ") + hipGetErrorString(e));
+}
+}  // namespace
+
+int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
+                  int32_t S, int32_t relu, int32_t input_grad) {
+    return nn_conv3_impl(x, w_master, b, y, cin, cout, S, relu, input_grad, nullptr, nullptr);
+}
+
+int gell_nn_conv3_pooled(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, uint16_t* pooled,
+                         int32_t cin, int32_t cout, int32_t S, int32_t relu, int32_t* in_kernel) {
+    if (!pooled) return sfail(nullptr, GELL_E_ARG, "null pooled");
+    return nn_conv3_impl(x, w_master, b, y, cin, cout, S, relu, 0, pooled, in_kernel);
 }
 
 // Weight gradient of one layer: dw_master (host, [cout][cin][27]) =

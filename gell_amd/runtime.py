@@ -161,6 +161,7 @@ def load_library() -> C.CDLL:
         "gell_surrogate_reset_budget": (C.c_int, [vp]),
         "gell_surrogate_gate_stats": (C.c_int, [vp, P(i64), P(i64), P(i64), P(C.c_double), P(C.c_double)]),
         "gell_nn_conv3": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, i32, i32]),
+        "gell_nn_conv3_pooled": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]),
         "gell_nn_conv3_wgrad": (C.c_int, [vp, vp, vp, i32, i32, i32]),
         "gell_nn_upconv3": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, i32]),
         "gell_io_last_error": (C.c_char_p, []),
@@ -588,6 +589,20 @@ def nn_conv3(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, cin
                                       int(input_grad))
     if st != 0:
         raise GellError(f"gell_nn_conv3: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
+
+
+def nn_conv3_pooled(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, pooled: DeviceBuffer, cin: int,
+                    cout: int, S: int, relu: bool = True) -> bool:
+    """nn_conv3 that also writes the 2x2x2 pooling of y; True when the
+    convolution's own epilogue formed it."""
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    flag = C.c_int32(0)
+    st = load_library().gell_nn_conv3_pooled(_ptr(x), w.ctypes.data, b.ctypes.data, _ptr(y), _ptr(pooled), cin, cout,
+                                             S, int(relu), C.byref(flag))
+    if st != 0:
+        raise GellError(f"gell_nn_conv3_pooled: {load_library().gell_surrogate_last_error(None).decode()} ({st})")
+    return bool(flag.value)
 
 
 def nn_upconv3(x: DeviceBuffer, w: np.ndarray, b: np.ndarray, y: DeviceBuffer, cin: int, cout: int, S: int,

@@ -490,7 +490,14 @@ int gell_surrogate_gate_stats(const gell_surrogate* sur, int64_t* checked, int64
  * reverses. Allocates; synchronises. */
 int gell_nn_conv3(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, int32_t cin, int32_t cout,
                   int32_t S, int32_t relu, int32_t input_grad);
-/* A layer that follows an upsampling (network layers 8, 10, 12: (cin, cout)
+/* gell_nn_conv3 (no input gradient) that also writes pooled ((S/2)^3 x cout)
+ * = the 2x2x2 mean of y, rounded to BF16 as the network's pooling rounds it:
+ * formed in the convolution's epilogue where its blocks allow (*in_kernel =
+ * 1), else by the pooling kernel. For checking the fused form. Allocates;
+ * synchronises. */
+int gell_nn_conv3_pooled(const uint16_t* x, const float* w_master, const float* b, uint16_t* y, uint16_t* pooled,
+                         int32_t cin, int32_t cout, int32_t S, int32_t relu, int32_t* in_kernel);
+/* A layer that follows an upsampling (network layers 7, 9, 11: (cin, cout)
  * = (128, 64), (64, 32), (32, 16)) as inference runs it: y ((2S)^3 x cout) =
  * act(conv3x3x3(up2(x)) + b) (+ skip when not NULL) computed on x's S^3 grid
  * (S a multiple of 4) as 8 per-parity 2x2x2 convolutions. x, skip, y device

@@ -71,6 +71,35 @@ def test_upconv_layer_equals_the_layer_on_the_upsampled_tensor(cin, cout, S):
         buf.free()
 
 
+@pytest.mark.parametrize("cin,cout,S", [(24, 16, 32), (32, 32, 64), (16, 32, 16), (64, 64, 32)])
+def test_conv_pooled_in_epilogue_equals_pooling_kernel(cin, cout, S):
+    """The pooling formed in a convolution's epilogue (lanes' shuffles and
+    pairs of z tiles) equals the pooling kernel on the stored output bit for
+    bit: the mean of the eight BF16 values summed in float32 in x, y, z order
+    and rounded to BF16. (64 -> 64 at 32^3 runs the K-split kernel, which does
+    not pool: the call pools with the kernel.)"""
+    rng = np.random.default_rng(cin + cout + S)
+    xb = to_bf16_bits(rng.normal(size=(S, S, S, cin)))
+    W = nno.bf16(rng.normal(0, np.sqrt(2 / (27 * cin)), size=(cout, cin, 27)))
+    b = rng.normal(0, 0.1, size=cout).astype(np.float32).astype(np.float64)
+    xd = dev(xb)
+    yd = rt.DeviceBuffer(S ** 3 * cout * 2)
+    pd = rt.DeviceBuffer((S // 2) ** 3 * cout * 2)
+    fused = rt.nn_conv3_pooled(xd, W, b, yd, pd, cin, cout, S, relu=True)
+    log(f"[nn] conv {cin}->{cout} at {S}^3: pooled in the epilogue {fused}")
+    assert fused == (cin != 64)
+    y = from_bits(yd.download(np.uint16, S ** 3 * cout)).reshape(S, S, S, cout).astype(np.float32)
+    h = S // 2
+    acc = np.zeros((h, h, h, cout), np.float32)
+    for d in range(8):  # x fastest, then y, then z; arrays are [z, y, x, c]
+        acc += y[(d >> 2)::2, ((d >> 1) & 1)::2, (d & 1)::2]
+    want = to_bf16_bits(acc * np.float32(0.125))
+    got = pd.download(np.uint16, h ** 3 * cout).reshape(h, h, h, cout)
+    assert np.array_equal(got, want)
+    for buf in (xd, yd, pd):
+        buf.free()
+
+
 @pytest.mark.parametrize("cin,cout", SHAPES)
 def test_conv_layer_forward_and_gradients(cin, cout):
     S = 16 if cin <= 64 else 8

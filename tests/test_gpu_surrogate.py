@@ -75,11 +75,21 @@ def test_tol0_gives_the_lod_substeps_bit_for_bit(fused):
         c.tune("lod_fused", fused)
     sur = rt.Surrogate(N, K)
     sur.set_params(weights())
+    m = np.zeros((N, N, N), bool)
+    m[2:N - 2, 2:N - 2, 2:N - 2] = True
+    m = m.ravel()
     for call in range(3):
         used, res = sur.advance(ctx, 0, 0.0)
         assert not used
-        for _ in range(K):
+        p0 = twin.field(0).astype(np.float64)
+        for k in range(K):
             twin.lod_substep(0)
+            if k == 0:
+                # rho, the first substep's relative increment (summed by the
+                # fused substep's z sweep, or from the two fields)
+                p1 = twin.field(0).astype(np.float64)
+                rho = np.sqrt(((p1 - p0)[m] ** 2).sum() / (p0 ** 2).sum())
+                assert abs(res - rho) <= 1e-6 * rho, (call, res, rho)
         assert np.array_equal(ctx.field(0).view(np.uint32), twin.field(0).view(np.uint32)), f"call {call}"
     st = sur.gate_stats()
     log(f"[surrogate] tol 0: {st}")
