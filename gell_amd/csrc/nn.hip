@@ -381,8 +381,7 @@ __global__ __launch_bounds__(kThreads) void conv3_fwd_splitk(const uint16_t* __r
 // + c, e3 = (ez 2 + ey) 2 + ex. Weights W'[p][o][k] (combine_up_weights).
 // Epilogue as conv3_fwd (bias, ReLU, BF16) at fine voxel 2c + p of the
 // (2S)^3 output; with `skip` the stored value is BF16(BF16(that) + skip),
-// the U-Net's skip connection rounded as a separate add would round it (the
-// skip values are loaded before the halo, so their latency overlaps it).
+// the U-Net's skip connection rounded as a separate add would round it.
 // ---------------------------------------------------------------------------
 template <int CIN, int COUT, int TZ, int CG>
 __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restrict__ d, const uint16_t* __restrict__ w,
@@ -396,7 +395,13 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
     constexpr int NCT = cpad(COUT) / 16;
     constexpr int VPW = 2 * TZ;  // two parities x TZ z slices
     static_assert(CIN % 32 == 0 && NCT % CG == 0, "tile shape");
+    // With every output channel in the block (CG = NCT) the block's output,
+    // 8 x 8 x 2 TZ fine voxels, is staged in LDS and leaves in whole rows
+    // (each lane's own voxels are 2 apart along x), the skip read the same way.
+    constexpr bool STAGE = CG == NCT;
+    constexpr int NF = 64 * 2 * TZ;  // fine voxels of the block
     __shared__ uint4 halo[HV * CH];
+    __shared__ uint2 ostage[STAGE ? NF * COUT / 4 : 1];
     const int zb = S / TZ;
     const int bz = blockIdx.z % zb, ct0 = (blockIdx.z / zb) * CG;
     const int cx0 = blockIdx.x * 4, cy0 = blockIdx.y * 4, cz0 = bz * TZ;
@@ -405,22 +410,13 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
     const int lx = r & 3, ly = r >> 2;
     const int F = 2 * S;
     // tile a: parity 2 wv + (a & 1), coarse z slice a >> 1; its output voxel
-    // for this lane, and the skip values there, loaded before anything waits
-    int64_t vox[VPW];
-    uint2 skv[VPW][CG];
-#pragma unroll
-    for (int a = 0; a < VPW; ++a) {
+    // for this lane
+    const auto vox_of = [&](int a) {
         const int p = 2 * wv + (a & 1);
         const int fx = 2 * (cx0 + lx) + (p & 1), fy = 2 * (cy0 + ly) + ((p >> 1) & 1),
                   fz = 2 * (cz0 + (a >> 1)) + (p >> 2);
-        vox[a] = ((int64_t)fz * F + fy) * F + fx;
-#pragma unroll
-        for (int c = 0; c < CG; ++c) {
-            const int o = (ct0 + c) * 16 + 4 * g;
-            skv[a][c] = (skip && o < COUT) ? *reinterpret_cast<const uint2*>(skip + vox[a] * COUT + o)
-                                           : make_uint2(0u, 0u);
-        }
-    }
+        return ((int64_t)fz * F + fy) * F + fx;
+    };
     stage_halo<HX, HY, HZ, CH>(d, S, S, S, cx0 - 1, cy0 - 1, cz0 - 1, halo);
     stage_wait();
     __syncthreads();
@@ -445,6 +441,7 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
         an[0][c] = *reinterpret_cast<const bf8*>(wr0 + c * 16 * KP);
         an[1][c] = *reinterpret_cast<const bf8*>(wr1 + c * 16 * KP);
     }
+#pragma unroll 2
     for (int s = 0; s < KSTEPS; ++s) {
         const int kb = s * 32, e3 = kb / CIN;  // window offset and chunk of k group 0
         const uint4* pb = lb + (((e3 >> 2) * HY + ((e3 >> 1) & 1)) * HX + (e3 & 1)) * CH + (kb % CIN) / 8;
@@ -469,6 +466,64 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
             for (int c = 0; c < CG; ++c)
                 acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a & 1][c], bf[a], acc[a][c], 0, 0, 0);
     }
+    if constexpr (STAGE) {
+        // BF16(act(y)) into the block's tile (fine voxel (lx, ly, lz) of the
+        // block at (lz 8 + ly) 8 + lx, COUT channels each) ...
+#pragma unroll
+        for (int a = 0; a < VPW; ++a) {
+            const int p = 2 * wv + (a & 1);
+            const int lv = ((2 * (a >> 1) + (p >> 2)) * 8 + 2 * ly + ((p >> 1) & 1)) * 8 + 2 * lx + (p & 1);
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                const int o = (ct0 + c) * 16 + 4 * g;
+                float t[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    t[i] = acc[a][c][i] + bias[o + i];
+                    if (relu) t[i] = fmaxf(t[i], 0.f);
+                }
+                ostage[(lv * COUT + o) / 4] = make_uint2(pack_bf2(t[0], t[1]), pack_bf2(t[2], t[3]));
+            }
+        }
+        __syncthreads();
+        // ... then out by 16-byte chunks, consecutive chunks along the fine
+        // rows, with the skip added: BF16(BF16(y) + skip), the rounding of a
+        // separate add
+        constexpr int CPV = COUT / 8;  // chunks per voxel
+        const uint4* st4 = reinterpret_cast<const uint4*>(ostage);
+        for (int q = threadIdx.x; q < NF * CPV; q += kThreads) {
+            const int lv = q / CPV, ch = q % CPV;
+            const int fxl = lv & 7, fyl = (lv >> 3) & 7, fzl = lv >> 6;
+            const int64_t gv = ((int64_t)(2 * cz0 + fzl) * F + 2 * cy0 + fyl) * F + 2 * cx0 + fxl;
+            uint4 v = st4[q];
+            if (skip) {
+                const uint4 sk = *reinterpret_cast<const uint4*>(skip + gv * COUT + ch * 8);
+                const uint32_t vw[4] = {v.x, v.y, v.z, v.w}, sw[4] = {sk.x, sk.y, sk.z, sk.w};
+                uint32_t ow[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float lo = __uint_as_float(vw[k] << 16) + __uint_as_float(sw[k] << 16);
+                    const float hi = __uint_as_float(vw[k] & 0xffff0000u) + __uint_as_float(sw[k] & 0xffff0000u);
+                    ow[k] = pack_bf2(lo, hi);
+                }
+                v = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+            }
+            *reinterpret_cast<uint4*>(y + gv * COUT + ch * 8) = v;
+        }
+        return;
+    }
+    // the skip values, all loads issued before the first is needed (here,
+    // not before the halo: holding them through the K loop took 16 registers
+    // and a wave per SIMD)
+    uint2 skv[VPW][CG];
+#pragma unroll
+    for (int a = 0; a < VPW; ++a)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) {
+            const int o = (ct0 + c) * 16 + 4 * g;
+            skv[a][c] = (skip && o < COUT) ? *reinterpret_cast<const uint2*>(skip + vox_of(a) * COUT + o)
+                                           : make_uint2(0u, 0u);
+        }
 #pragma unroll
     for (int a = 0; a < VPW; ++a) {
 #pragma unroll
@@ -491,7 +546,7 @@ __global__ __launch_bounds__(kThreads) void upconv_fwd(const uint16_t* __restric
                 const float u3 = __uint_as_float(pk.y & 0xffff0000u) + __uint_as_float(sk.y & 0xffff0000u);
                 pk = make_uint2(pack_bf2(u0, u1), pack_bf2(u2, u3));
             }
-            *reinterpret_cast<uint2*>(y + vox[a] * COUT + o) = pk;
+            *reinterpret_cast<uint2*>(y + vox_of(a) * COUT + o) = pk;
         }
     }
 }

@@ -41,13 +41,15 @@ def close_bf16(got, want):
     return np.all(err <= 2.0 ** -7 * np.abs(want) + 2.0 ** -9 * 1e-2 * scale), float(err.max() / scale)
 
 
-@pytest.mark.parametrize("cin,cout,S", [(128, 64, 8), (64, 32, 8), (32, 16, 16)])
+@pytest.mark.parametrize("cin,cout,S", [(128, 64, 8), (64, 32, 8), (32, 16, 16), (64, 32, 32)])
 def test_upconv_layer_equals_the_layer_on_the_upsampled_tensor(cin, cout, S):
     """The layers after an upsampling run on the coarse grid as 8 per-parity
     2x2x2 convolutions with summed FP32 taps (rounded to BF16 once); against
     the oracle's 3x3x3 convolution of the formed (2S)^3 upsampling with the
     unrounded weights, ReLU, BF16, + skip, BF16: within the rounding of the
-    combined weights (2^-9 relative each) and of the outputs."""
+    combined weights (2^-9 relative each) and of the outputs. (32, 16, 16)
+    and (64, 32, 32) take the form whose blocks hold every output channel and
+    stage their output in LDS; the others one channel tile per block."""
     rng = np.random.default_rng(cin * 7 + cout)
     xb = to_bf16_bits(rng.normal(size=(S, S, S, cin)))
     x = from_bits(xb)
