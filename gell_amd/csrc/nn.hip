@@ -1061,8 +1061,11 @@ hipError_t conv_t(const uint16_t* x, const uint16_t* w, const float* b, uint16_t
     if (pooled_done) *pooled_done = false;
     if (X % kTX || Y % kTY || Z % TZ) return hipErrorInvalidValue;
     if constexpr (CIN % 32 == 0) {
-        // coarse levels (grids up to 32 per side): K split over the waves
-        if (X <= 32 && Y <= 32) {
+        // coarse levels: K split over the waves up to 32^3 grids, and at 64^3
+        // unless the output is to be pooled too (the plain kernel's blocks
+        // pool their own output there: 34.8 + 6.0 -> 36 us, against the split
+        // kernel's 31 us alone)
+        if ((X <= 32 && Y <= 32) || (X <= 64 && Y <= 64 && !pooled)) {
             constexpr int CGS = NCT % 2 == 0 ? 2 : 1;
             hipLaunchKernelGGL((conv3_fwd_splitk<CIN, COUT, CGS>), dim3(X / kTX, Y / kTY, Z * (NCT / CGS)),
                                dim3(kThreads), 0, st, x, w, b, y, X, Y, Z, relu);
