@@ -907,11 +907,10 @@ __global__ __launch_bounds__(256) void incr_finalize(const double* __restrict__ 
     }
 }
 
-// Input features on the half-resolution grid (X = N / 2), 24 channels per
-// coarse voxel: q = (p1 - p0) / s at its 8 fine voxels split into a BF16 high
-// part (channels 0..7) and the BF16 rounding of the remainder (8..15), then
-// the uptake rate times rate_scale at the same 8 (16..23). Fine voxel
-// d = (z & 1) 4 + (y & 1) 2 + (x & 1).
+// Input features on the half-resolution grid (X = N / 2), 16 channels per
+// coarse voxel: q = (p1 - p0) / s at its 8 fine voxels rounded to BF16
+// (channels 0..7), then the uptake rate times rate_scale at the same 8
+// (8..15). Fine voxel d = (z & 1) 4 + (y & 1) 2 + (x & 1).
 __global__ void features(const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ rate,
                          int N, const float* __restrict__ norm, float rate_scale, uint16_t* __restrict__ f) {
     const int X = N / 2;
@@ -930,26 +929,14 @@ __global__ void features(const float* __restrict__ p0, const float* __restrict__
         const float2 r = *reinterpret_cast<const float2*>(rate + o);
         qv[2 * q] = (b.x - a.x) * inv_s;
         qv[2 * q + 1] = (b.y - a.y) * inv_s;
-        rv[2 * q] = r.x;
-        rv[2 * q + 1] = r.y;
+        rv[2 * q] = r.x * rate_scale;
+        rv[2 * q + 1] = r.y * rate_scale;
     }
-    uint16_t h[24];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        h[d] = f2bf(qv[d]);
-        h[8 + d] = f2bf(qv[d] - bf2f(h[d]));
-        h[16 + d] = f2bf(rv[d] * rate_scale);
-    }
-    uint4 pk[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        pk[q].x = (uint32_t)h[8 * q + 0] | ((uint32_t)h[8 * q + 1] << 16);
-        pk[q].y = (uint32_t)h[8 * q + 2] | ((uint32_t)h[8 * q + 3] << 16);
-        pk[q].z = (uint32_t)h[8 * q + 4] | ((uint32_t)h[8 * q + 5] << 16);
-        pk[q].w = (uint32_t)h[8 * q + 6] | ((uint32_t)h[8 * q + 7] << 16);
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) reinterpret_cast<uint4*>(f + v * 24)[q] = pk[q];
+    uint4 pk[2];
+    pk[0] = make_uint4(pack_bf2(qv[0], qv[1]), pack_bf2(qv[2], qv[3]), pack_bf2(qv[4], qv[5]), pack_bf2(qv[6], qv[7]));
+    pk[1] = make_uint4(pack_bf2(rv[0], rv[1]), pack_bf2(rv[2], rv[3]), pack_bf2(rv[4], rv[5]), pack_bf2(rv[6], rv[7]));
+    reinterpret_cast<uint4*>(f + v * 16)[0] = pk[0];
+    reinterpret_cast<uint4*>(f + v * 16)[1] = pk[1];
 }
 
 // Full-resolution output: out[fine] = p0[fine] + s out_scale head[coarse][sub],
@@ -1128,7 +1115,7 @@ hipError_t conv3(const uint16_t* x, const uint16_t* w, const float* b, uint16_t*
                  int Z, int relu, hipStream_t st, uint16_t* pooled, bool* pooled_done) {
 #define GELL_NN_CASE(a, c) \
     if (cin == a && cout == c) return conv_t<a, c>(x, w, b, y, X, Y, Z, relu, st, pooled, pooled_done);
-    GELL_NN_CASE(24, 16) GELL_NN_CASE(16, 24) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32)
+    GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32)
     GELL_NN_CASE(32, 64) GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64)
     GELL_NN_CASE(64, 32) GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8) GELL_NN_CASE(8, 16)
 #undef GELL_NN_CASE
@@ -1155,7 +1142,7 @@ hipError_t conv3_weight_grad(const uint16_t* x, const uint16_t* dy, float* scrat
                              int Y, int Z, int accumulate, hipStream_t st) {
 #define GELL_NN_CASE(a, c) \
     if (cin == a && cout == c) return wgrad_t<a, c>(x, dy, scratch, out, X, Y, Z, accumulate, st);
-    GELL_NN_CASE(24, 16) GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
+    GELL_NN_CASE(16, 16) GELL_NN_CASE(16, 32) GELL_NN_CASE(32, 32) GELL_NN_CASE(32, 64)
     GELL_NN_CASE(64, 64) GELL_NN_CASE(64, 128) GELL_NN_CASE(128, 128) GELL_NN_CASE(128, 64) GELL_NN_CASE(64, 32)
     GELL_NN_CASE(32, 16) GELL_NN_CASE(16, 8)
 #undef GELL_NN_CASE

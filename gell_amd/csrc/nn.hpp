@@ -62,8 +62,8 @@ hipError_t increment_norms(const float* p0, const float* p1, int N, double* part
 // substep's last kernel).
 hipError_t finalize_norms(const double* part, int nblk, int N, float* norm, double* sums, hipStream_t st,
                           double* host_sums = nullptr);
-// p0, p1 (N^3) + uptake map -> 24-channel features on the (N/2)^3 grid: r / s
-// as BF16 high and low parts, rate * rate_scale (8 fine voxels each)
+// p0, p1 (N^3) + uptake map -> 16-channel features on the (N/2)^3 grid: r / s
+// and rate * rate_scale, rounded to BF16 (8 fine voxels each)
 hipError_t input_features(const float* p0, const float* p1, const float* rate, int N, const float* norm,
                           float rate_scale, uint16_t* f, hipStream_t st);
 // (N/2)^3 x 8 head -> N^3 field: p0 + s out_scale head in the interior, p1 on

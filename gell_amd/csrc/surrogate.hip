@@ -16,9 +16,9 @@
 // The network, on the half-resolution grid (N/2)^3, is a four-level U-Net of
 // 3x3x3 convolutions (BF16 on the matrix cores, FP32 accumulation):
 //
-//   features(r / s, rate) 24 ch: r / s (as BF16 high + low parts) and rate
-//                   at the 8 fine voxels of each coarse one
-//   level 1 (N/2):  L0 24->16 = e1                        pool
+//   features(r / s, rate) 16 ch: r / s and rate (BF16) at the 8 fine
+//                   voxels of each coarse one
+//   level 1 (N/2):  L0 16->16 = e1                        pool
 //   level 2 (N/4):  L1 16->32, L2 32->32 = e2             pool
 //   level 3 (N/8):  L3 32->64, L4 64->64 = e3             pool
 //   level 4 (N/16): L5 64->128, L6 128->128 = e4          up
@@ -54,7 +54,7 @@ struct LayerSpec {
 };
 
 const LayerSpec kLayers[] = {
-    {24, 16, 1, true},  {16, 32, 2, true},   {32, 32, 2, true},  {32, 64, 3, true}, {64, 64, 3, true},
+    {16, 16, 1, true},  {16, 32, 2, true},   {32, 32, 2, true},  {32, 64, 3, true}, {64, 64, 3, true},
     {64, 128, 4, true}, {128, 128, 4, true}, {128, 64, 3, true}, {64, 64, 3, true}, {64, 32, 2, true},
     {32, 32, 2, true},  {32, 16, 1, true},   {16, 8, 1, false},
 };
@@ -372,7 +372,7 @@ int gell_surrogate_create(const gell_surrogate_config* cfg, void* stream, gell_s
         if (after_up(l)) SA(salloc(s, &s->wu[l], up_weight_count(kLayers[l].cin, kLayers[l].cout)));
     }
     SA(salloc(s, &s->zero_bias, 128));
-    SA(tensor(s, s->f0, S1, 24));
+    SA(tensor(s, s->f0, S1, 16));
     SA(tensor(s, s->e1, S1, 16));
     SA(tensor(s, s->p1, S2, 16));
     SA(tensor(s, s->a2, S2, 32));

@@ -471,9 +471,12 @@ int gell_surrogate_adam(gell_surrogate* sur, float lr);
  * seen). Between those checks a prediction is made and kept (*used = 1) only
  * while the estimated errors (1.5 ratio rho) of the predictions kept in the
  * current window add up to at most `tol`, relative L2; otherwise the other
- * K - 1 substeps run (*used = 0) and the network does not. tol = 0 makes
- * every call exact (bit for bit K substeps). ctx must have grid N and run on
- * the surrogate's stream. Synchronises. */
+ * K - 1 substeps run (*used = 0). tol = 0 makes every call exact (bit for
+ * bit K substeps). On checks, and after a call that kept its prediction, the
+ * network is queued before rho is read (a call that then falls back has run
+ * it for nothing). ctx must have grid N and run on the surrogate's stream.
+ * The host waits for rho; the call returns with the rest of its work queued
+ * on the stream (a check waits for all of it). */
 int gell_surrogate_advance(gell_surrogate* sur, gell_ctx* ctx, int32_t s, float tol, int32_t* used, float* residual);
 /* Start a new error-budget window (e.g. at each biology step). */
 int gell_surrogate_reset_budget(gell_surrogate* sur);

@@ -23,7 +23,7 @@ from __future__ import annotations
 import numpy as np
 
 LAYERS = [  # (cin, cout, level, relu)
-    (24, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True), (64, 64, 3, True),
+    (16, 16, 1, True), (16, 32, 2, True), (32, 32, 2, True), (32, 64, 3, True), (64, 64, 3, True),
     (64, 128, 4, True), (128, 128, 4, True), (128, 64, 3, True), (64, 64, 3, True), (64, 32, 2, True),
     (32, 32, 2, True), (32, 16, 1, True), (16, 8, 1, False),
 ]
@@ -127,20 +127,20 @@ def increment_scale(p0, p1):
 
 
 def features(p0, p1, rate, rate_scale, rnd=ident):
-    """24-channel input on the half grid (fine arrays indexed [z, y, x]):
-    q = (p1 - p0) / s at the 8 fine voxels of each coarse voxel as a rounded
-    high part and the rounded remainder, then rate times rate_scale at the
-    same 8 (fine voxel (z & 1) 4 + (y & 1) 2 + (x & 1)). Without rounding the
-    remainder channels are zero. With rounding (the device's emulation) q is
-    formed in float32, as the device forms it."""
+    """16-channel input on the half grid (fine arrays indexed [z, y, x]):
+    q = (p1 - p0) / s at the 8 fine voxels of each coarse voxel, then rate
+    times rate_scale at the same 8 (fine voxel (z & 1) 4 + (y & 1) 2 +
+    (x & 1)), each rounded. With rounding (the device's emulation) q and the
+    scaled rate are formed in float32, as the device forms them."""
     _, inv = increment_scale(p0, p1)
     if rnd is ident:
         q = fine_to_head((np.asarray(p1, np.float64) - np.asarray(p0, np.float64)) * inv)
+        rr = fine_to_head(np.asarray(rate, np.float64) * rate_scale)
     else:
         d = (np.float32(p1) - np.float32(p0)) * np.float32(inv)
         q = fine_to_head(d.astype(np.float64))
-    hi = rnd(q)
-    return np.concatenate([hi, rnd(q - hi), rnd(fine_to_head(rate * rate_scale))], axis=-1)
+        rr = fine_to_head((np.float32(rate) * np.float32(rate_scale)).astype(np.float64))
+    return np.concatenate([rnd(q), rnd(rr)], axis=-1)
 
 
 def head_to_fine(h):

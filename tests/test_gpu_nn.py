@@ -15,7 +15,7 @@ from helpers import log, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(24, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 64), (64, 32),
+SHAPES = [(16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 64), (64, 32),
           (32, 16), (16, 8)]
 
 
@@ -73,7 +73,7 @@ def test_upconv_layer_equals_the_layer_on_the_upsampled_tensor(cin, cout, S):
         buf.free()
 
 
-@pytest.mark.parametrize("cin,cout,S", [(24, 16, 32), (32, 32, 64), (16, 32, 16), (64, 64, 32)])
+@pytest.mark.parametrize("cin,cout,S", [(16, 16, 32), (32, 32, 64), (16, 32, 16), (64, 64, 32)])
 def test_conv_pooled_in_epilogue_equals_pooling_kernel(cin, cout, S):
     """The pooling formed in a convolution's epilogue (lanes' shuffles and
     pairs of z tiles) equals the pooling kernel on the stored output bit for
