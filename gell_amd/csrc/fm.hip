@@ -911,32 +911,111 @@ __device__ __forceinline__ void cell_force_occ(const float4* __restrict__ nb, co
     }
 }
 
+// The pair test of forcekernel (FMkernel.cuh:89-99) decided without the
+// square root. distance = sqrtf(x), x = (float)(ss + 0.0001), is correctly
+// rounded, so for a positive float t: distance < t  <=>  sqrt(x) < m, with m
+// the midpoint of t and its float predecessor. m has at most 26 significant
+// bits, so m^2 is exact in double and never a float: x < m^2 decides it with
+// no tie. A NaN x passes, as it passes the reference's `distance >= thres`.
+struct CutoffSq {
+    float thres, ss_max;
+    double m2;
+    __device__ CutoffSq() { set(25.f); }  // Tumor_CC_Max_Dist (FMkernel.cuh:72)
+    __device__ void set(float t) {
+        thres = t;
+        ss_max = t * t * 1.0000153f;
+        const double m = ((double)__uint_as_float(__float_as_uint(t) - 1u) + (double)t) * 0.5;
+        m2 = m * m;
+    }
+    __device__ bool below(float x) const { return !((double)x >= m2); }
+};
+
+// Acceptance half of force_pair: true when the pair contributes a force; the
+// cutoff carries over exactly as in force_pair.
+__device__ __forceinline__ bool pair_accepts(float4 o, float p1x, float p1y, float p1z, float r1, CutoffSq& cut) {
+    if (__float_as_uint(o.w) == kDeadTag) return false;
+    const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
+    const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+    if (ss >= cut.ss_max) return false;
+    const float x = (float)((double)ss + 0.0001);
+    if (!cut.below(x)) return false;  // distance >= thres
+    const float rsum = o.w + r1;
+    const float R = rsum > 0.01f ? rsum : 0.01f;
+    cut.set((float)((double)R * 1.25));
+    return cut.below(x);  // distance < 1.25 R
+}
+
+// Force of an accepted pair (FMkernel.cuh:94-109), i.e. what force_pair adds
+// after its tests passed: the same operations in the same order, with
+// thres = 1.25 R of this pair.
+__device__ __forceinline__ float3 pair_force(float4 o, float p1x, float p1y, float p1z, float r1) {
+    const float ddx = p1x - o.x, ddy = p1y - o.y, ddz = p1z - o.z;
+    const float ss = ddx * ddx + ddy * ddy + ddz * ddz;
+    const float distance = sqrtf((float)((double)ss + 0.0001));
+    const float rsum = o.w + r1;
+    const float R = rsum > 0.01f ? rsum : 0.01f;
+    const float thres = (float)((double)R * 1.25);
+    const float qr = 1.f - distance / R;
+    const float qa = 1.f - distance / thres;
+    const float coefr = (distance <= R) ? 10.f * (qr * qr / distance) : 0.f;
+    const float coefa = (distance <= thres) ? 0.4f * (qa * qa / distance) : 0.f;
+    const float cc = coefr - coefa;
+    return make_float3(ddx * cc, ddy * cc, ddz * cc);
+}
+
+__device__ __forceinline__ float lane_f(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
 // Rank variant of the occupancy walk: the stencil's (at most eight) 16-byte
 // word records give the occupied voxels and their ranks, and voxel q's
 // candidates are the slots [cs[rank], cs[rank + 1]) -- its live cells in
 // ascending order with, at most, dead cells after them (skipped by the dead
 // tag, as the reference skips dead cells inside a range, FMkernel.cuh:86).
-// Visiting order and pair arithmetic as in the other variants.
+// Visiting order as in the other variants: x outer, z inner, slots ascending.
+//
+// The whole wave calls it (live == false: no candidates). Each lane tests its
+// candidates in order with pair_accepts, one per iteration of a loop that
+// runs until every lane is done. A pair with zero displacement (the cell
+// itself) adds +-0 * cc with cc finite, which leaves a force that started at
+// +0 unchanged (a float sum that starts at +0 never becomes -0), so only its
+// cutoff update counts. Every other accepted pair goes to a queue of the
+// wave's accepted pairs (64 entries in LDS: slot and owner lane, plus a bit
+// per entry in the owner's mask). When an iteration's pairs would overflow it
+// (before they join) and at the end, the queue is drained: lane k computes entry k's force
+// (pair_force), then every owner adds its entries' forces in queue order,
+// which is its own acceptance order. The sums are therefore force_pair's,
+// while the pair arithmetic runs once per 64 accepted pairs of the wave
+// instead of at every candidate position any of its lanes reaches.
 __device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, const uint4* __restrict__ occr,
-                                                const int32_t* __restrict__ cs, int32_t N, float dx, float p1x,
-                                                float p1y, float p1z, float r1, float& fx, float& fy, float& fz) {
-    const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
-    Cutoff cut;  // thres = Tumor_CC_Max_Dist, carried across pairs (FMkernel.cuh:72)
+                                                const int32_t* __restrict__ cs, int32_t N, float dx, bool live,
+                                                float p1x, float p1y, float p1z, float r1, int2* wq, float& fx,
+                                                float& fy, float& fz) {
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(occr), (short)0, 0x7FFFFFFF, 0x00020000);
     const __amdgpu_buffer_rsrc_t rc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(cs), (short)0, 0x7FFFFFFF, 0x00020000);
-    const StencilCodes sc = stencil_codes(px, py, pz);
-    uint32_t mask = occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16);
-    const auto pick = [](const uint32_t (&v)[3], uint32_t o) { return o == 0 ? v[0] : (o == 1 ? v[1] : v[2]); };
-    if (mask == 0) return;
+    const int lane = (int)__lane_id();
+    // the stencil's dilated coordinates as scalars: selected by value below
+    // (an array picked by a runtime index would live in scratch)
+    uint32_t cx0 = 0, cx1 = 0, cx2 = 0, cy0 = 0, cy1 = 0, cy2 = 0, cz0 = 0, cz1 = 0, cz2 = 0, mask = 0;
+    if (live) {
+        const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
+        const StencilCodes sc = stencil_codes(px, py, pz);
+        mask = occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16);
+        cx0 = sc.cx[0], cx1 = sc.cx[1], cx2 = sc.cx[2];
+        cy0 = sc.cy[0], cy1 = sc.cy[1], cy2 = sc.cy[2];
+        cz0 = sc.cz[0], cz1 = sc.cz[1], cz2 = sc.cz[2];
+    }
     // [first, end) candidate slots of stencil voxel q. Its word's record is
     // read again (a cache hit: the eight were just loaded) rather than picked
     // from the registers above, which a runtime index would move to scratch.
     const auto range_of = [&](uint32_t q) {
-        const uint32_t xo = q / 9, yo = (q / 3) % 3, zo = q % 3;
-        const uint32_t code = pick(sc.cx, xo) | pick(sc.cy, yo) | pick(sc.cz, zo);
+        const uint32_t xo = (q * 57u) >> 9, r = q - 9u * xo, yo = (r * 11u) >> 5, zo = r - 3u * yo;
+        const uint32_t code = (xo == 0 ? cx0 : 0u) | (xo == 1 ? cx1 : 0u) | (xo == 2 ? cx2 : 0u) |
+                              (yo == 0 ? cy0 : 0u) | (yo == 1 ? cy1 : 0u) | (yo == 2 ? cy2 : 0u) |
+                              (zo == 0 ? cz0 : 0u) | (zo == 1 ? cz1 : 0u) | (zo == 2 ? cz2 : 0u);
         const uint4 rec =
             __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ro, (int)((code >> 6) * 16u), 0, 0));
         const int32_t rk = voxel_rank(rec, code);
@@ -944,25 +1023,93 @@ __device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, c
         const int32_t e = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, rk * 4 + 4, 0, 0);
         return make_int2(s, e);
     };
-    int2 cur = range_of((uint32_t)__builtin_ctz(mask));
-    mask &= mask - 1;
-    int j = cur.x;
-    float4 o = load_nb(rs, j);
-    while (true) {
-        const bool more = mask != 0;
-        const int2 nxt = more ? range_of((uint32_t)__builtin_ctz(mask)) : make_int2(0, 0);
+    // current voxel's candidates [j, e); the next voxel's range nxt (nxt.y ==
+    // 0: none); o = the candidate at j, loaded one iteration ahead
+    int j = 0, e = 0;
+    int2 nxt = make_int2(0, 0);
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (mask != 0) {
+        const int2 cur = range_of((uint32_t)__builtin_ctz(mask));
         mask &= mask - 1;
-        const int e = cur.y - 1;
-        for (; j <= e; ++j) {
-            // FMkernel.cuh:89-109, one candidate's load ahead of its evaluation
-            const float4 ahead = load_nb(rs, j < e ? j + 1 : (more ? nxt.x : j));
-            force_pair(o, p1x, p1y, p1z, r1, cut, fx, fy, fz);
-            o = ahead;
-        }
-        if (!more) break;
-        cur = nxt;
         j = cur.x;
+        e = cur.y;
+        if (mask != 0) {
+            nxt = range_of((uint32_t)__builtin_ctz(mask));
+            mask &= mask - 1;
+        }
+        o = load_nb(rs, j);
     }
+    CutoffSq cut;
+    uint64_t mine = 0;  // queue entries this lane owns
+    int queued = 0;     // entries in the queue (the same in every lane)
+    const auto drain = [&]() {
+        // lane k: entry k's force
+        const int2 ent = wq[lane < queued ? lane : 0];
+        const int src = lane < queued ? ent.y : lane;
+        const float qx = lane_f(p1x, src), qy = lane_f(p1y, src), qz = lane_f(p1z, src), qr = lane_f(r1, src);
+        float3 f = make_float3(0.f, 0.f, 0.f);
+        if (lane < queued) f = pair_force(load_nb(rs, ent.x), qx, qy, qz, qr);
+        // owners add their entries' forces in queue order
+        while (__any(mine != 0)) {
+            const int k = mine != 0 ? (int)__builtin_ctzll(mine) : lane;
+            const float gx = lane_f(f.x, k), gy = lane_f(f.y, k), gz = lane_f(f.z, k);
+            if (mine != 0) {
+                fx += gx;
+                fy += gy;
+                fz += gz;
+                mine &= mine - 1;
+            }
+        }
+        queued = 0;
+    };
+    // One candidate per lane: oc is the candidate at j, on receives the one
+    // after it. The loop below alternates the two registers, so a load is
+    // consumed one iteration after it was issued (a copy of it at the end of
+    // each iteration would wait for it at once). False when every lane is done.
+    const auto step = [&](float4& oc, float4& on) {
+        const bool active = j < e;
+        if (!__any(active)) return false;
+        bool enq = false;
+        const int slot = j;
+        if (active) {
+            // on the voxel's last candidate, the range of the voxel after the
+            // next one first (its loads are then older than the candidate's)
+            const bool last = j + 1 >= e;
+            int2 nn = nxt;
+            if (last) {
+                nn = make_int2(0, 0);
+                if (mask != 0) {
+                    nn = range_of((uint32_t)__builtin_ctz(mask));
+                    mask &= mask - 1;
+                }
+            }
+            // the next candidate: this voxel's next slot, else the next voxel's first
+            on = load_nb(rs, !last ? j + 1 : (nxt.y > 0 ? nxt.x : j));
+            // FMkernel.cuh:86-99
+            if (pair_accepts(oc, p1x, p1y, p1z, r1, cut))
+                enq = !(p1x - oc.x == 0.f && p1y - oc.y == 0.f && p1z - oc.z == 0.f);
+            ++j;
+            if (last) {
+                j = nxt.x;
+                e = nxt.y;
+                nxt = nn;
+            }
+        }
+        const uint64_t bal = __ballot(enq);
+        if (queued + __popcll(bal) > 64) drain();  // older entries first: their forces come first
+        if (enq) {
+            const int pos = queued + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            wq[pos] = make_int2(slot, lane);
+            mine |= 1ull << pos;
+        }
+        queued += __popcll(bal);
+        return true;
+    };
+    float4 o2 = o;
+    while (step(o, o2) && step(o2, o)) {
+    }
+    if (queued > 0) drain();
 }
 
 // movementkernel, FMkernel.cuh:130-158: AB2 step of a live cell; outside the
@@ -1138,11 +1285,11 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
     const float4 me = in_range ? load_nb(nb_rsrc(nb), (int32_t)i) : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
     const bool live = __float_as_uint(me.w) != kDeadTag;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (live) {
-        if (RANK)
-            cell_force_rank(nb, occr, cs, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
-        else
-            cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
+    if (RANK) {
+        __shared__ int2 wq[kBlock];  // each wave's queue of accepted pairs
+        cell_force_rank(nb, occr, cs, N, dx, live, me.x, me.y, me.z, me.w, wq + (threadIdx.x & ~63u), fx, fy, fz);
+    } else if (live) {
+        cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
     }
     if (!STAGED) {
         CellRegs c{};
