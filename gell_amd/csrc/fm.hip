@@ -967,12 +967,14 @@ __device__ __forceinline__ float lane_f(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
 
-// Rank variant of the occupancy walk: the stencil's (at most eight) 16-byte
-// word records give the occupied voxels and their ranks, and voxel q's
-// candidates are the slots [cs[rank], cs[rank + 1]) -- its live cells in
+// The occupancy walk of variants 7-9 (forcekernel's stencil loop,
+// FMkernel.cuh:74-112): the stencil's occupied voxels come from the
+// occupancy words, visited in the reference's order (x outer, z inner, slots
+// ascending). RANK (variant 9): a voxel's candidates are the slots
+// [cs[rank], cs[rank + 1]) of the rank-compacted list -- its live cells in
 // ascending order with, at most, dead cells after them (skipped by the dead
-// tag, as the reference skips dead cells inside a range, FMkernel.cuh:86).
-// Visiting order as in the other variants: x outer, z inner, slots ascending.
+// tag, as the reference skips dead cells inside a range, FMkernel.cuh:86);
+// otherwise (7, 8) its Morton-indexed range se[code] = [first, last].
 //
 // The whole wave calls it (live == false: no candidates). Each lane tests its
 // candidates in order with pair_accepts, one per iteration of a loop that
@@ -981,16 +983,19 @@ __device__ __forceinline__ float lane_f(float v, int src) {
 // +0 unchanged (a float sum that starts at +0 never becomes -0), so only its
 // cutoff update counts. Every other accepted pair goes to a queue of the
 // wave's accepted pairs (64 entries in LDS: slot and owner lane, plus a bit
-// per entry in the owner's mask). When an iteration's pairs would overflow it
-// (before they join) and at the end, the queue is drained: lane k computes entry k's force
-// (pair_force), then every owner adds its entries' forces in queue order,
-// which is its own acceptance order. The sums are therefore force_pair's,
-// while the pair arithmetic runs once per 64 accepted pairs of the wave
-// instead of at every candidate position any of its lanes reaches.
-__device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, const uint4* __restrict__ occr,
-                                                const int32_t* __restrict__ cs, int32_t N, float dx, bool live,
-                                                float p1x, float p1y, float p1z, float r1, int2* wq, float& fx,
-                                                float& fy, float& fz) {
+// per entry in the owner's mask). When an iteration's pairs would overflow
+// it (before they join) and at the end, the queue is drained: lane k
+// computes entry k's force (pair_force), then every owner adds its entries'
+// forces in queue order, which is its own acceptance order. The sums are
+// therefore force_pair's, while the pair arithmetic runs once per 64
+// accepted pairs of the wave instead of at every candidate position any of
+// its lanes reaches.
+template <bool RANK>
+__device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb, const int2* __restrict__ se,
+                                                  const uint32_t* __restrict__ occ, const uint4* __restrict__ occr,
+                                                  const int32_t* __restrict__ cs, int32_t N, float dx, bool live,
+                                                  float p1x, float p1y, float p1z, float r1, int2* wq, float& fx,
+                                                  float& fy, float& fz) {
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(occr), (short)0, 0x7FFFFFFF, 0x00020000);
@@ -1003,7 +1008,8 @@ __device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, c
     if (live) {
         const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
         const StencilCodes sc = stencil_codes(px, py, pz);
-        mask = occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16);
+        mask = RANK ? occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16)
+                    : occ_mask(occ, N, px, py, pz, sc);
         cx0 = sc.cx[0], cx1 = sc.cx[1], cx2 = sc.cx[2];
         cy0 = sc.cy[0], cy1 = sc.cy[1], cy2 = sc.cy[2];
         cz0 = sc.cz[0], cz1 = sc.cz[1], cz2 = sc.cz[2];
@@ -1016,6 +1022,10 @@ __device__ __forceinline__ void cell_force_rank(const float4* __restrict__ nb, c
         const uint32_t code = (xo == 0 ? cx0 : 0u) | (xo == 1 ? cx1 : 0u) | (xo == 2 ? cx2 : 0u) |
                               (yo == 0 ? cy0 : 0u) | (yo == 1 ? cy1 : 0u) | (yo == 2 ? cy2 : 0u) |
                               (zo == 0 ? cz0 : 0u) | (zo == 1 ? cz1 : 0u) | (zo == 2 ? cz2 : 0u);
+        if (!RANK) {
+            const int2 r = se[code];  // [first, last]
+            return make_int2(r.x, r.y + 1);
+        }
         const uint4 rec =
             __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ro, (int)((code >> 6) * 16u), 0, 0));
         const int32_t rk = voxel_rank(rec, code);
@@ -1285,12 +1295,9 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
     const float4 me = in_range ? load_nb(nb_rsrc(nb), (int32_t)i) : make_float4(0.f, 0.f, 0.f, __uint_as_float(kDeadTag));
     const bool live = __float_as_uint(me.w) != kDeadTag;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (RANK) {
-        __shared__ int2 wq[kBlock];  // each wave's queue of accepted pairs
-        cell_force_rank(nb, occr, cs, N, dx, live, me.x, me.y, me.z, me.w, wq + (threadIdx.x & ~63u), fx, fy, fz);
-    } else if (live) {
-        cell_force_occ(nb, se, occ, N, dx, me.x, me.y, me.z, me.w, fx, fy, fz);
-    }
+    __shared__ int2 wq[kBlock];  // each wave's queue of accepted pairs
+    cell_force_queued<RANK>(nb, se, occ, occr, cs, N, dx, live, me.x, me.y, me.z, me.w, wq + (threadIdx.x & ~63u), fx,
+                            fy, fz);
     if (!STAGED) {
         CellRegs c{};
         if (in_range) {
