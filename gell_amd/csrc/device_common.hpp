@@ -186,23 +186,21 @@ __device__ __forceinline__ void uptake_accumulate_wave(bool take, int32_t v, flo
     out_code = code;
     const int lane = (int)(threadIdx.x & 63);
     const int32_t prev = __shfl_up(code, 1, 64), next = __shfl_down(code, 1, 64);
-    const bool head = lane == 0 || prev != code;
-    const uint64_t tails = __ballot(lane == 63 || next != code);
-    // last lane of this lane's run
-    const int last = (int)__builtin_ctzll(tails & ~((1ull << lane) - 1ull));
+    const bool tail = lane == 63 || next != code;
+    const uint64_t heads = __ballot(lane == 0 || prev != code);
+    // first lane of this lane's run
+    const int first = 63 - (int)__builtin_clzll(heads & ((2ull << lane) - 1ull));
     for (int s = 0; s < u.S; ++s) {
         long long q = take ? to_fixed(cell_uptake(V, phase, dx, u.uptake[s], u.dead_factor[s])) : 0;
-        // inclusive prefix sum over the wave
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
+        // inclusive scan within runs (after the step with offset d a lane holds
+        // the sum over the last 2d lanes of its run), only as many steps as the
+        // wave's longest run needs: at ~1 cell per voxel, none or one
+        for (int d = 1; __any(lane - d >= first); d <<= 1) {
             const long long y = __shfl_up(q, d, 64);
-            if (lane >= d) q += y;
+            if (lane - d >= first) q += y;
         }
-        const long long at_last = __shfl(q, last, 64);
-        const long long before = __shfl_up(q, 1, 64);
-        const long long run = at_last - (lane > 0 ? before : 0);
-        if (head && code >= 0)
-            atomicAdd(reinterpret_cast<unsigned long long*>(u.acc[s] + code), (unsigned long long)run);
+        if (tail && code >= 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(u.acc[s] + code), (unsigned long long)q);
     }
 }
 
