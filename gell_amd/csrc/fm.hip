@@ -872,6 +872,71 @@ __device__ __forceinline__ uint32_t occ_mask(const uint32_t* __restrict__ occ, i
     return mask;
 }
 
+// The same 27-bit mask as occ_mask, built a row at a time. In a word (an
+// aligned 4^3 block) the bit of local voxel (x, y, z) is x0 + 2 y0 + 4 z0 +
+// 8 x1 + 16 y1 + 32 z1, so shifting the word right by a row's x and y part
+// (the low six bits of its dilated x and y codes, masked to 0x1B) leaves the
+// row's four z voxels at bits 0, 4, 32 and 36. The two z-words of a row (the
+// blocks of the first and the last stencil z) give a line of eight z voxels,
+// from which the stencil's three are one bit-field extract. Stencil voxels
+// outside the grid read a word of the stencil (the loads stay at in-grid
+// coordinates) and are cleared at the end.
+__device__ __forceinline__ uint32_t occ_mask_rows(const uint32_t* __restrict__ occ, int32_t N, int32_t px, int32_t py,
+                                                  int32_t pz, const StencilCodes& c, uint32_t word_bytes) {
+    const __amdgpu_buffer_rsrc_t ro =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(occ), (short)0, 0x7FFFFFFF, 0x00020000);
+    const bool vx0 = px > 0, vx2 = px + 1 < N, vy0 = py > 0, vy2 = py + 1 < N, vz0 = pz > 0, vz2 = pz + 1 < N;
+    const uint32_t xs = vx0 ? c.cx[0] : c.cx[1], xe = vx2 ? c.cx[2] : c.cx[1];
+    const uint32_t ys = vy0 ? c.cy[0] : c.cy[1], ye = vy2 ? c.cy[2] : c.cy[1];
+    const uint32_t zs = vz0 ? c.cz[0] : c.cz[1], ze = vz2 ? c.cz[2] : c.cz[1];
+    uint64_t W[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const uint32_t wi = ((a ? xe : xs) | (b ? ye : ys) | (d ? ze : zs)) >> 6;
+                const uint2 w =
+                    __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(ro, (int)(wi * word_bytes), 0, 0));
+                W[a][b][d] = ((uint64_t)w.y << 32) | w.x;
+            }
+    // z voxels 0..3 of a shifted word (bits 0, 4, 32, 36) as bits 0..3
+    const auto nibble = [](uint64_t w) {
+        const uint32_t t = ((uint32_t)w & 0x11u) | (((uint32_t)(w >> 32) & 0x11u) << 2);  // z0 z2 z1 z3 at 0 2 4 6
+        return (t | (t >> 3)) & 0xFu;
+    };
+    // which of the two x (y) blocks stencil column 1 and 2 lie in (column 0
+    // lies in the first whenever it is inside the grid)
+    const bool a1 = (c.cx[1] >> 6) != (xs >> 6), a2 = (c.cx[2] >> 6) != (xs >> 6);
+    const bool b1 = (c.cy[1] >> 6) != (ys >> 6), b2 = (c.cy[2] >> 6) != (ys >> 6);
+    // bit of stencil z 0 in the z line below (the line starts one bit up, so
+    // that a first stencil z below the grid starts at bit 0)
+    const uint32_t start = vz0 ? (uint32_t)((pz - 1) & 3) + 1u : 0u;
+    uint32_t rows = 0;
+#pragma unroll
+    for (int xo = 0; xo < 3; ++xo) {
+        const bool a = xo == 0 ? false : (xo == 1 ? a1 : a2);
+        const uint64_t X00 = a ? W[1][0][0] : W[0][0][0], X01 = a ? W[1][0][1] : W[0][0][1];
+        const uint64_t X10 = a ? W[1][1][0] : W[0][1][0], X11 = a ? W[1][1][1] : W[0][1][1];
+#pragma unroll
+        for (int yo = 0; yo < 3; ++yo) {
+            const bool b = yo == 0 ? false : (yo == 1 ? b1 : b2);
+            const uint32_t base = (c.cx[xo] | c.cy[yo]) & 0x1Bu;
+            const uint32_t line = (nibble((b ? X10 : X00) >> base) | (nibble((b ? X11 : X01) >> base) << 4)) << 1;
+            rows |= __builtin_amdgcn_ubfe(line, start, 3u) << (9 * xo + 3 * yo);
+        }
+    }
+    uint32_t valid = 0x7FFFFFFu;
+    if (!vx0) valid &= ~0x1FFu;
+    if (!vx2) valid &= ~0x7FC0000u;
+    if (!vy0) valid &= ~0x1C0E07u;
+    if (!vy2) valid &= ~0x70381C0u;
+    if (!vz0) valid &= ~0x1249249u;
+    if (!vz2) valid &= ~0x4924924u;
+    return rows & valid;
+}
+
 // Occupancy-mask variant: only the ranges of the stencil voxels that hold a
 // live cell are loaded (at ~0.1 cells per voxel, about 4 of 27). Each range is
 // loaded while the previous voxel's candidates are walked, and the load one
@@ -1008,8 +1073,8 @@ __device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb,
     if (live) {
         const int32_t px = trunc_index(p1x / dx), py = trunc_index(p1y / dx), pz = trunc_index(p1z / dx);
         const StencilCodes sc = stencil_codes(px, py, pz);
-        mask = RANK ? occ_mask(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16)
-                    : occ_mask(occ, N, px, py, pz, sc);
+        mask = RANK ? occ_mask_rows(reinterpret_cast<const uint32_t*>(occr), N, px, py, pz, sc, 16)
+                    : occ_mask_rows(occ, N, px, py, pz, sc, 8);
         cx0 = sc.cx[0], cx1 = sc.cx[1], cx2 = sc.cx[2];
         cy0 = sc.cy[0], cy1 = sc.cy[1], cy2 = sc.cy[2];
         cz0 = sc.cz[0], cz1 = sc.cz[1], cz2 = sc.cz[2];
