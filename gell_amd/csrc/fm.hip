@@ -1031,6 +1031,9 @@ __device__ __forceinline__ float3 pair_force(float4 o, float p1x, float p1y, flo
 __device__ __forceinline__ float lane_f(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
+__device__ __forceinline__ uint32_t lane_u(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
 
 // The occupancy walk of variants 7-9 (forcekernel's stencil loop,
 // FMkernel.cuh:74-112): the stencil's occupied voxels come from the
@@ -1055,12 +1058,13 @@ __device__ __forceinline__ float lane_f(float v, int src) {
 // therefore force_pair's, while the pair arithmetic runs once per 64
 // accepted pairs of the wave instead of at every candidate position any of
 // its lanes reaches.
+constexpr int kBalCap = 448;  // stencil voxels of a wave whose ranges are looked up before the walk
 template <bool RANK>
 __device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb, const int2* __restrict__ se,
                                                   const uint32_t* __restrict__ occ, const uint4* __restrict__ occr,
                                                   const int32_t* __restrict__ cs, int32_t N, float dx, bool live,
-                                                  float p1x, float p1y, float p1z, float r1, int2* wq, float& fx,
-                                                  float& fy, float& fz) {
+                                                  float p1x, float p1y, float p1z, float r1, int2* wq, int2* rq,
+                                                  float& fx, float& fy, float& fz) {
     const __amdgpu_buffer_rsrc_t rs = nb_rsrc(nb);
     const __amdgpu_buffer_rsrc_t ro =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(occr), (short)0, 0x7FFFFFFF, 0x00020000);
@@ -1082,11 +1086,14 @@ __device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb,
     // [first, end) candidate slots of stencil voxel q. Its word's record is
     // read again (a cache hit: the eight were just loaded) rather than picked
     // from the registers above, which a runtime index would move to scratch.
-    const auto range_of = [&](uint32_t q) {
+    const auto code_of = [](uint32_t q, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t y0, uint32_t y1,
+                            uint32_t y2, uint32_t z0, uint32_t z1, uint32_t z2) {
         const uint32_t xo = (q * 57u) >> 9, r = q - 9u * xo, yo = (r * 11u) >> 5, zo = r - 3u * yo;
-        const uint32_t code = (xo == 0 ? cx0 : 0u) | (xo == 1 ? cx1 : 0u) | (xo == 2 ? cx2 : 0u) |
-                              (yo == 0 ? cy0 : 0u) | (yo == 1 ? cy1 : 0u) | (yo == 2 ? cy2 : 0u) |
-                              (zo == 0 ? cz0 : 0u) | (zo == 1 ? cz1 : 0u) | (zo == 2 ? cz2 : 0u);
+        return (xo == 0 ? x0 : 0u) | (xo == 1 ? x1 : 0u) | (xo == 2 ? x2 : 0u) | (yo == 0 ? y0 : 0u) |
+               (yo == 1 ? y1 : 0u) | (yo == 2 ? y2 : 0u) | (zo == 0 ? z0 : 0u) | (zo == 1 ? z1 : 0u) |
+               (zo == 2 ? z2 : 0u);
+    };
+    const auto range_at = [&](uint32_t code) {
         if (!RANK) {
             const int2 r = se[code];  // [first, last]
             return make_int2(r.x, r.y + 1);
@@ -1098,20 +1105,63 @@ __device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb,
         const int32_t e = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, rk * 4 + 4, 0, 0);
         return make_int2(s, e);
     };
+    // Ranges looked up before the walk (RANK with an LDS region rq, when the
+    // wave's stencil voxels fit): every lane lists its occupied stencil
+    // voxels (owner lane, q) in order, the wave's lanes then resolve the list
+    // 64 entries at a time, and a lane's walk reads its ranges from rq. A lane
+    // reaches its next voxel at a different iteration of the walk than its
+    // neighbours, so looked up inside the walk the chain (word record, rank,
+    // cell starts) runs in almost every iteration for some lane.
+    bool listed = false;
+    int t_next = 0, t_end = 0;
+    if (RANK && rq) {
+        const int v = __popc(mask);
+        int incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += y;
+        }
+        const int V = __shfl(incl, 63, 64);
+        listed = V <= kBalCap;
+        if (listed) {
+            uint16_t* keys = reinterpret_cast<uint16_t*>(rq + kBalCap);
+            t_next = incl - v;
+            t_end = incl;
+            int t = t_next;
+            for (uint32_t mm = mask; mm != 0; mm &= mm - 1) keys[t++] = (uint16_t)(((uint32_t)lane << 5) | __builtin_ctz(mm));
+            for (int k0 = 0; k0 < V; k0 += 64) {
+                const int k = k0 + lane;
+                const uint32_t key = k < V ? (uint32_t)keys[k] : ((uint32_t)lane << 5);
+                const int ow = (int)(key >> 5);
+                const uint32_t code =
+                    code_of(key & 31u, lane_u(cx0, ow), lane_u(cx1, ow), lane_u(cx2, ow), lane_u(cy0, ow),
+                            lane_u(cy1, ow), lane_u(cy2, ow), lane_u(cz0, ow), lane_u(cz1, ow), lane_u(cz2, ow));
+                if (k < V) rq[k] = range_at(code);
+            }
+        }
+    }
+    // the range of this lane's next occupied stencil voxel ({0, 0}: none)
+    const auto next_range = [&]() {
+        int2 r = make_int2(0, 0);
+        if (listed) {
+            if (t_next < t_end) r = rq[t_next++];
+        } else if (mask != 0) {
+            r = range_at(code_of((uint32_t)__builtin_ctz(mask), cx0, cx1, cx2, cy0, cy1, cy2, cz0, cz1, cz2));
+            mask &= mask - 1;
+        }
+        return r;
+    };
     // current voxel's candidates [j, e); the next voxel's range nxt (nxt.y ==
     // 0: none); o = the candidate at j, loaded one iteration ahead
     int j = 0, e = 0;
     int2 nxt = make_int2(0, 0);
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
     if (mask != 0) {
-        const int2 cur = range_of((uint32_t)__builtin_ctz(mask));
-        mask &= mask - 1;
+        const int2 cur = next_range();
         j = cur.x;
         e = cur.y;
-        if (mask != 0) {
-            nxt = range_of((uint32_t)__builtin_ctz(mask));
-            mask &= mask - 1;
-        }
+        nxt = next_range();
         o = load_nb(rs, j);
     }
     CutoffSq cut;
@@ -1151,13 +1201,7 @@ __device__ __forceinline__ void cell_force_queued(const float4* __restrict__ nb,
             // next one first (its loads are then older than the candidate's)
             const bool last = j + 1 >= e;
             int2 nn = nxt;
-            if (last) {
-                nn = make_int2(0, 0);
-                if (mask != 0) {
-                    nn = range_of((uint32_t)__builtin_ctz(mask));
-                    mask &= mask - 1;
-                }
-            }
+            if (last) nn = next_range();
             // the next candidate: this voxel's next slot, else the next voxel's first
             on = load_nb(rs, !last ? j + 1 : (nxt.y > 0 ? nxt.x : j));
             // FMkernel.cuh:86-99
@@ -1353,7 +1397,7 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
                                                             int32_t* __restrict__ list,
                                                             const int32_t* __restrict__ np) {
     n = count_of(n, np);
-    __shared__ uint32_t rec[STAGED ? kBlock * kRecDw : 1];
+    __shared__ __align__(16) uint32_t rec[STAGED ? kBlock * kRecDw : 1];
     const int64_t b0 = (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * kBlock;
     const int64_t i = b0 + threadIdx.x;
     const bool in_range = i < n;
@@ -1361,8 +1405,12 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
     const bool live = __float_as_uint(me.w) != kDeadTag;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     __shared__ int2 wq[kBlock];  // each wave's queue of accepted pairs
-    cell_force_queued<RANK>(nb, se, occ, occr, cs, N, dx, live, me.x, me.y, me.z, me.w, wq + (threadIdx.x & ~63u), fx,
-                            fy, fz);
+    // RANK with staged records: the staging LDS holds each wave's listed
+    // stencil ranges during the walk (kBalCap ranges and keys in its quarter)
+    static_assert(kBalCap * (sizeof(int2) + sizeof(uint16_t)) <= 64 * kRecDw * sizeof(uint32_t), "rq fits");
+    int2* rq = (RANK && STAGED) ? reinterpret_cast<int2*>(rec + (threadIdx.x & ~63u) * kRecDw) : nullptr;
+    cell_force_queued<RANK>(nb, se, occ, occr, cs, N, dx, live, me.x, me.y, me.z, me.w, wq + (threadIdx.x & ~63u), rq,
+                            fx, fy, fz);
     if (!STAGED) {
         CellRegs c{};
         if (in_range) {
@@ -1374,7 +1422,9 @@ __global__ __launch_bounds__(kBlock) void fm_force_move_xcd(const gell_cell* __r
         return;
     }
     // the tile's records, coalesced, into LDS (after the walk, so that the
-    // walk's registers are free again); none past a device-resident count
+    // walk's registers are free again, and with RANK after every wave's
+    // walk, whose ranges were kept there); none past a device-resident count
+    if (RANK) __syncthreads();
     const int m = (int)max<int64_t>(0, min<int64_t>((int64_t)n - b0, kBlock));
     const int ndw = m * kRecDw;
     const int nq = ndw / 4;  // whole 16-byte chunks; an odd m leaves one 8-byte tail
